@@ -1,0 +1,140 @@
+"""Throughput of the NSSurfacelet analyze+synthesize pair on one MI355X.
+
+Workload: NSSurfacelet(scales = 3, direction_levels = [2, 2, 3]) (defaults
+bernstein_order = 12, halfband_order = 1, transition = 1, tight) on a
+128 x 128 x 128 Float32 volume of i.i.d. N(0, 1) voxels (numpy PCG64, seed
+20260), 49 subbands.  One step = analyze! + synthesize! with a prebuilt
+workspace; the input and the coefficients stay resident in HBM.
+
+metric: voxel-subbands per second = n1 n2 n3 S / (seconds per step).
+roofline: algorithmic HBM bytes of one step, 8 N (S + 1) (read x, write S
+subbands; read S subbands, write x), over the step's duration measured with
+HIP events on the stream every kernel is launched on, against 8 TB/s.
+cpu_baseline: the FP32 CPU restatement of the reference's schedule
+(oracle/cpu_schedule.py) on the host cores, on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "surfacelets.jl_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+DIMS = (128, 128, 128)
+SEED = 20260
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="approximate CPU work budget of the cpu_baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(x_np, budget_s):
+    from oracle import ns as O
+    from oracle.cpu_schedule import CPUSchedule
+    workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    workers = max(1, min(workers, 16))
+    t = O.NSParams(3, (2, 2, 3))
+    sch = CPUSchedule(t, DIMS, workers)
+    S = sch.S
+    out = [None] * S
+    x = x_np.astype(np.float32)
+    sch.analyze(x, out)          # warm-up: plans, page faults
+    sch.synthesize(out)
+    pairs, t0 = 0, time.perf_counter()
+    while True:
+        sch.analyze(x, out)
+        sch.synthesize(out)
+        pairs += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = (time.perf_counter() - t0) / pairs
+    N = int(np.prod(DIMS))
+    return {"value": N * S / dt, "unit": "voxel·subbands/s", "cores": workers, "kind": "port",
+            "sample": f"{pairs} analyze+synthesize pairs of the same 128^3 workload (FP32 scipy.fft, "
+                      f"{workers} worker threads, one single-threaded FFT per task), {dt * 1e3:.0f} ms per pair"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the surfacelet path here runs on a single MI355X "
+              "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    global np
+    import numpy as np
+    import torch
+    import surfhip
+    from surfhip import ns as NS
+    from surfhip._arrays import from_numpy, to_numpy
+
+    dev = "cuda"
+    torch.cuda.set_device(0)
+    t = NS.NSSurfacelet(scales=3, direction_levels=[2, 2, 3])
+    x_np = np.random.default_rng(SEED).standard_normal(DIMS).astype(np.float32)
+    x = from_numpy(x_np, dev)
+    ws = NS.make_workspace(t, x)
+    S = NS.nsubbands(t, DIMS)
+    c = NS.similar_coefficients(t, x)
+    y = torch.empty_like(x)
+    N = int(np.prod(DIMS))
+
+    def step():
+        NS.analyze_(c, ws, x)
+        NS.synthesize_(y, ws, c)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    rel = float(torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x))
+
+    bytes_per_step = 8 * N * (S + 1)
+    achieved = bytes_per_step / (ms * 1e-3) / 1e9
+    result = {
+        "metric": "analyze+synth voxel·subbands/s, 128³ FP32, 49 subbands; % HBM peak",
+        "value": N * S / (ms * 1e-3),
+        "unit": "voxel·subbands/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: i.i.d. N(0,1) Float32 volume, numpy PCG64 seed 20260",
+        "config": {"workload": "NSSurfacelet(scales=3, direction_levels=[2,2,3]) analyze!+synthesize!, "
+                               "128x128x128 Float32, 49 subbands, prebuilt workspace",
+                   "dims": list(DIMS), "subbands": S, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "scope": "one analyze!+synthesize! step (all its kernels, one stream), "
+                              "algorithmic bytes 8 N (S+1)"},
+        "check": {"reconstruction_rel_l2": rel},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(x_np, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

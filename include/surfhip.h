@@ -1,0 +1,157 @@
+/*
+ * surfhip.h -- C ABI of libsurfhip, the MI355X (gfx950) analyze/synthesize
+ * path of the 3-D surfacelet transforms.
+ *
+ * The entry points replace what the reference's device extension does today
+ * (ext/SurfaceletsAdaptExt.jl:25 `adapt_workspace`, and the broadcasts plus
+ * vendor-FFT calls it routes every analyze!/synthesize! through).  A host
+ * binding (Julia `ccall`, Python ctypes) keeps the reference API
+ * (src/transforms/ns_surfacelet.jl:88-173,504-641 and
+ * src/transforms/surfacelet.jl:126-199,292-464) and calls these.
+ *
+ * Conventions
+ * - Volumes are column-major n1 x n2 x n3 (axis 1 contiguous), as in the
+ *   reference.  Real data is float32; complex data is interleaved float32
+ *   pairs (ComplexF32) and runs as two real channels.
+ * - NS coefficients are (n1, n2, n3, S) column-major: subband i (1-based) at
+ *   element offset (i-1)*n1*n2*n3.  Decimated coefficients are one flat vector
+ *   with subbands in subband_layout order (see surfhip_s_layout).
+ * - Every pointer passed to analyze/synthesize/stream calls is a DEVICE
+ *   pointer.  Calls enqueue work on the plan's HIP stream and return without
+ *   synchronising; the caller synchronises the stream before reading results.
+ * - A plan serves one call at a time (as the reference's workspaces:
+ *   src/transforms/ns_surfacelet.jl:51-53, src/transforms/surfacelet.jl:40-42).
+ * - Status codes: SURFHIP_OK, or an error whose message surfhip_last_error
+ *   returns (thread-local).  ARG maps to the reference's ArgumentError, DIM to
+ *   DimensionMismatch.
+ */
+#ifndef SURFHIP_H
+#define SURFHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SURFHIP_OK = 0,
+  SURFHIP_ERR_ARG = 1,     /* ArgumentError */
+  SURFHIP_ERR_DIM = 2,     /* DimensionMismatch */
+  SURFHIP_ERR_DEVICE = 3,  /* HIP runtime failure */
+  SURFHIP_ERR_OOM = 4      /* device allocation failure */
+};
+
+enum { SURFHIP_F32 = 0, SURFHIP_C64 = 1 };
+
+#define SURFHIP_MAX_SCALES 16
+
+/* NSSurfacelet parameters: src/types/transforms.jl:86-113. */
+typedef struct {
+  int32_t scales;
+  int32_t direction_levels[SURFHIP_MAX_SCALES]; /* coarsest scale first */
+  int32_t bernstein_order;
+  int32_t halfband_order;
+  double transition;
+  int32_t tight;
+} surfhip_ns_params;
+
+/* Surfacelet (decimated) parameters: src/types/transforms.jl:222-261.
+ * levels[j][K][m]: checkerboard stages along axis m inside hourglass K at
+ * scale j (coarsest first), diagonal zero -- the level matrix of
+ * src/types/transforms.jl:270-284. */
+typedef struct {
+  int32_t scales;
+  int32_t levels[SURFHIP_MAX_SCALES][3][3];
+  double pyramid_ratio;     /* 1, 1.5 or 2 */
+  int32_t checkerboard_order;
+  int32_t diamond_length;
+  double kaiser_beta;
+  int32_t diamond_power;
+} surfhip_s_params;
+
+typedef struct surfhip_ns_plan surfhip_ns_plan;
+typedef struct surfhip_s_plan surfhip_s_plan;
+
+/* Message of the last failed call on this thread; returns its full length. */
+size_t surfhip_last_error(char *buf, size_t n);
+
+/* Library version string, e.g. "surfhip 0.1 gfx950". */
+const char *surfhip_version(void);
+
+/* ------------------------------------------------------------ NSSurfacelet */
+
+/* Subband count, host only: src/transforms/ns_surfacelet.jl:527
+ * (nsubbands) and src/types/transforms.jl:157. */
+int surfhip_ns_nsubbands(const surfhip_ns_params *p, int32_t *out);
+
+/* make_workspace: src/transforms/ns_surfacelet.jl:88-123,165-215.  Builds
+ * the FP64 tables on the host (rounded once to FP32), uploads them and
+ * allocates the plan's buffers on the current device.  `stream` is a
+ * hipStream_t (NULL: the default stream).  Errors as the reference:
+ * "volume size must be positive", "too small for J scales". */
+int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p,
+                           int32_t n1, int32_t n2, int32_t n3, int32_t dtype,
+                           void *stream);
+int surfhip_ns_plan_destroy(surfhip_ns_plan *plan);
+int surfhip_ns_set_stream(surfhip_ns_plan *plan, void *stream);
+
+/* analyze!: src/transforms/ns_surfacelet.jl:551-566.  x: n1 n2 n3 volume,
+ * coeffs: (n1, n2, n3, S). */
+int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs);
+
+/* synthesize!: src/transforms/ns_surfacelet.jl:568-581. */
+int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x);
+
+/* Streaming analysis, analyze(f, t, x): src/transforms/ns_surfacelet.jl:
+ * 474-479,620-624.  stream_begin computes the spectrum of x; each
+ * stream_subband(i) then writes subband i (1-based, subband_layout order)
+ * into `band` (one volume).  The reference calls f for i = 2..S, then 1. */
+int surfhip_ns_stream_begin(surfhip_ns_plan *plan, const void *x);
+int surfhip_ns_stream_subband(surfhip_ns_plan *plan, int32_t i, void *band);
+
+/* Streaming synthesis, synthesize!(f, x, t): src/transforms/ns_surfacelet.jl:
+ * 483-493,637-641.  inverse_begin clears the accumulator; accumulate adds
+ * subband i; finish writes the volume. */
+int surfhip_ns_stream_inverse_begin(surfhip_ns_plan *plan);
+int surfhip_ns_stream_accumulate(surfhip_ns_plan *plan, int32_t i, const void *band);
+int surfhip_ns_stream_finish(surfhip_ns_plan *plan, void *x);
+
+/* Host only: the analysis mask of subband i (1-based) on the rfft grid,
+ * (n1/2+1, n2, n3) column-major, as the device evaluates it (the same FP32
+ * tables and expression).  The inspection counterpart of hourglass_mask /
+ * ndfb_mask (src/directional/hourglass.jl:264-270). */
+int surfhip_ns_mask_host(const surfhip_ns_params *p, int32_t n1, int32_t n2,
+                         int32_t n3, int32_t i, float *out);
+
+/* ------------------------------------------------------ Surfacelet (dec.) */
+
+/* Size check with the reference's message (including "the nearest length
+ * that fits"): src/transforms/surfacelet.jl:230-269.  Host only. */
+int surfhip_s_check_size(const surfhip_s_params *p, int32_t n1, int32_t n2, int32_t n3);
+
+/* Subband count and total coefficient count, host only:
+ * src/transforms/surfacelet.jl:275-290,310. */
+int surfhip_s_counts(const surfhip_s_params *p, int32_t n1, int32_t n2,
+                     int32_t n3, int32_t *nsub, int64_t *ncoef);
+
+/* subband_layout sizes (3 per subband) and element offsets (1 per subband)
+ * of the flat coefficient vector, host only. */
+int surfhip_s_layout(const surfhip_s_params *p, int32_t n1, int32_t n2,
+                     int32_t n3, int64_t *sizes3, int64_t *offsets, int32_t cap);
+
+int surfhip_s_plan_create(surfhip_s_plan **out, const surfhip_s_params *p,
+                          int32_t n1, int32_t n2, int32_t n3, int32_t dtype,
+                          void *stream);
+int surfhip_s_plan_destroy(surfhip_s_plan *plan);
+int surfhip_s_set_stream(surfhip_s_plan *plan, void *stream);
+
+/* analyze! / synthesize!: src/transforms/surfacelet.jl:446-464. */
+int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs);
+int surfhip_s_synthesize(surfhip_s_plan *plan, const void *coeffs, void *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SURFHIP_H */

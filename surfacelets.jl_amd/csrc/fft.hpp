@@ -1,0 +1,203 @@
+// Batched mixed-radix Stockham FFT on LDS-resident pencils (gfx950).
+//
+// A block holds G pencils of length n in LDS, element (t, g) at [t * G + g],
+// so that consecutive lanes touch consecutive words.  Each pass of radix R
+// reads a[j + r n/R], applies the twiddles w^(r (j mod Ns) n/(Ns R)), runs an
+// R-point DFT in registers and writes b[(j/Ns) Ns R + (j mod Ns) + r Ns]
+// (decimation in time, self-sorting: no bit reversal), ping-ponging two LDS
+// buffers.  Radices 8, 4, 2, 3, 5 are hand-written; other primes up to
+// FFT_MAX_RADIX use a direct DFT.  Twiddles w^m = exp(-2 pi i m / n) are made
+// in FP64 on the host and rounded once to FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#define FFT_MAX_PASSES 24
+#define FFT_MAX_RADIX 32
+
+namespace surfhip {
+
+struct FftPlan1D {
+  int32_t n;
+  int32_t npass;
+  int32_t tw;  // offset of this length's n twiddles in the pool
+  int32_t radix[FFT_MAX_PASSES];
+};
+
+// Host: factor n (radix 8 first, then 4, 2, 3, 5 and remaining primes) and
+// append its twiddles to `pool`.  Throws for a prime factor > FFT_MAX_RADIX.
+FftPlan1D make_fft_plan(int n, std::vector<float2> &pool);
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+// a * (-i) for the forward transform, a * (+i) for the inverse.
+template <int DIR> __device__ __forceinline__ float2 rot90(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+// Twiddle w^m of the forward (DIR = -1) or inverse (DIR = +1) transform.
+template <int DIR> __device__ __forceinline__ float2 twid(const float2 *tw, int m) {
+  float2 w = tw[m];
+  return DIR < 0 ? w : make_float2(w.x, -w.y);
+}
+
+template <int DIR> __device__ __forceinline__ void dft2(float2 *v) {
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int DIR> __device__ __forceinline__ void dft4(float2 *v) {
+  float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  float2 t2 = cadd(v[1], v[3]), t3 = rot90<DIR>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+
+template <int DIR> __device__ __forceinline__ void dft8(float2 *v) {
+  const float c = 0.70710678118654752440f;
+  float2 e[4] = {v[0], v[2], v[4], v[6]};
+  float2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  // w8^1, w8^2, w8^3 of the forward transform: (c,-c), (0,-1), (-c,-c).
+  float2 o1 = DIR < 0 ? make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x))
+                      : make_float2(c * (o[1].x - o[1].y), c * (o[1].y + o[1].x));
+  float2 o2 = rot90<DIR>(o[2]);
+  float2 o3 = DIR < 0 ? make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y))
+                      : make_float2(-c * (o[3].x + o[3].y), c * (o[3].x - o[3].y));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <int DIR> __device__ __forceinline__ void dft3(float2 *v) {
+  const float s3 = 0.86602540378443864676f;
+  float2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+  float2 m = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+  // forward: X1 = m - i s3 d, X2 = m + i s3 d
+  float2 r = rot90<DIR>(cscale(d, s3));
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <int DIR> __device__ __forceinline__ void dft5(float2 *v) {
+  // cos/sin of 2 pi/5 and 4 pi/5.
+  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+  float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+  float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  float2 m1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+  float2 m2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+  // forward: X1 = m1 - i (s1 b1 + s2 b2), X2 = m2 - i (s2 b1 - s1 b2)
+  float2 n1 = rot90<DIR>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+  float2 n2 = rot90<DIR>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+  v[0] = cadd(v[0], cadd(a1, a2));
+  v[1] = cadd(m1, n1);
+  v[4] = csub(m1, n1);
+  v[2] = cadd(m2, n2);
+  v[3] = csub(m2, n2);
+}
+
+// Direct R-point DFT for other primes, twiddles from the length-n table.
+template <int DIR> __device__ __noinline__ void dft_generic(float2 *v, int R, const float2 *tw, int n) {
+  float2 out[FFT_MAX_RADIX];
+  const int step = n / R;
+  for (int s = 0; s < R; ++s) {
+    float2 acc = v[0];
+    int e = 0;
+    for (int r = 1; r < R; ++r) {
+      e += s;
+      if (e >= R) e -= R;
+      acc = cadd(acc, cmul(v[r], twid<DIR>(tw, e * step)));
+    }
+    out[s] = acc;
+  }
+  for (int s = 0; s < R; ++s) v[s] = out[s];
+}
+
+template <int R, int DIR> __device__ __forceinline__ void fft_pass_fixed(
+    const float2 *__restrict__ a, float2 *__restrict__ b, int n, int G, int Ns, const float2 *tw,
+    int tid, int nthreads) {
+  const int nR = n / R;
+  const int tstep = n / (Ns * R);
+  for (int w = tid; w < G * nR; w += nthreads) {
+    const int g = w % G, j = w / G;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = a[(j + r * nR) * G + g];
+    const int k = j % Ns;
+    if (k) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<DIR>(tw, r * k * tstep));
+    }
+    if (R == 2) dft2<DIR>(v);
+    else if (R == 3) dft3<DIR>(v);
+    else if (R == 4) dft4<DIR>(v);
+    else if (R == 5) dft5<DIR>(v);
+    else if (R == 8) dft8<DIR>(v);
+    const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) b[(idxD + r * Ns) * G + g] = v[r];
+  }
+}
+
+template <int DIR> __device__ __noinline__ void fft_pass_generic(
+    const float2 *__restrict__ a, float2 *__restrict__ b, int n, int R, int G, int Ns,
+    const float2 *tw, int tid, int nthreads) {
+  const int nR = n / R;
+  const int tstep = n / (Ns * R);
+  for (int w = tid; w < G * nR; w += nthreads) {
+    const int g = w % G, j = w / G;
+    float2 v[FFT_MAX_RADIX];
+    for (int r = 0; r < R; ++r) v[r] = a[(j + r * nR) * G + g];
+    const int k = j % Ns;
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<DIR>(tw, (r * k * tstep) % n));
+    dft_generic<DIR>(v, R, tw, n);
+    const int idxD = (j / Ns) * Ns * R + k;
+    for (int r = 0; r < R; ++r) b[(idxD + r * Ns) * G + g] = v[r];
+  }
+}
+
+// Transform the G pencils in `a` (LDS), using `b` as the other half of the
+// ping-pong pair; returns the buffer holding the result.  `tw` holds the
+// plan's n twiddles (LDS or global).  All threads of the block must call it.
+template <int DIR> __device__ float2 *lds_fft(float2 *a, float2 *b, const FftPlan1D &pl, int G,
+                                              const float2 *tw, int tid, int nthreads) {
+  int Ns = 1;
+  const int n = pl.n;
+  for (int p = 0; p < pl.npass; ++p) {
+    const int R = pl.radix[p];
+    switch (R) {
+      case 8: fft_pass_fixed<8, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
+      case 4: fft_pass_fixed<4, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
+      case 2: fft_pass_fixed<2, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
+      case 3: fft_pass_fixed<3, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
+      case 5: fft_pass_fixed<5, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
+      default: fft_pass_generic<DIR>(a, b, n, R, G, Ns, tw, tid, nthreads); break;
+    }
+    __syncthreads();
+    float2 *t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+}  // namespace surfhip

@@ -1,0 +1,185 @@
+"""NSSurfacelet on the GPU through libsurfhip vs the FP64 oracle.
+
+Tolerances: FP32 results against the FP64 oracle, relative L2 over all
+subbands <= 1e-5 and per subband <= 1e-4 of that
+subband's norm or 1e-6 of the input norm, whichever is larger (subbands with
+almost no energy carry only rounding).  Structural properties the reference
+asserts bit for bit (streaming == materialised, complex == re + i im,
+workspace reuse) are asserted bit for bit here.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ns as O
+
+pytestmark = pytest.mark.gpu
+
+surfhip = pytest.importorskip("surfhip")
+from surfhip import ns as N  # noqa: E402
+from surfhip._arrays import from_numpy, to_numpy  # noqa: E402
+
+RTOL = 1e-5
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return "cuda"
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))
+
+
+def _check_coeffs(c_gpu, c_ref, xnorm):
+    S = len(c_ref)
+    got = to_numpy(c_gpu)
+    num = sum(np.sum(np.abs(got[..., i] - c_ref[i]) ** 2) for i in range(S))
+    den = sum(np.sum(np.abs(c_ref[i]) ** 2) for i in range(S))
+    assert np.sqrt(num / den) <= RTOL, np.sqrt(num / den)
+    for i in range(S):
+        err = np.linalg.norm((got[..., i] - c_ref[i]).ravel())
+        ref = np.linalg.norm(c_ref[i].ravel())
+        assert err <= max(1e-4 * ref, 1e-6 * xnorm), (i, err, ref)
+
+
+CASES = [
+    ((16, 16, 16), 2, (1, 2), 6, True),
+    ((12, 20, 8), 3, (1, 1, 2), 6, True),
+    ((15, 20, 8), 3, (1, 1, 2), 6, True),
+    ((9, 7, 5), 1, (1,), 6, True),
+    ((16, 12, 20), 2, (1, 2), 6, False),
+    ((26, 26, 26), 2, (2, 2), 12, True),
+    ((32, 24, 40), 3, (2, 2, 3), 12, True),
+]
+
+
+@pytest.mark.parametrize("dims,J,L,bo,tight", CASES)
+def test_ns_analyze_synthesize_match_oracle(dims, J, L, bo, tight):
+    dev = _dev()
+    print(f"case dims={dims} J={J} L={L} bo={bo} tight={tight}", flush=True)
+    rng = np.random.default_rng(hash((dims, J)) % 2 ** 32)
+    x = rng.standard_normal(dims).astype(np.float32)
+    t = N.NSSurfacelet(scales=J, direction_levels=list(L), bernstein_order=bo, tight=tight)
+    to = O.NSParams(J, tuple(L), bernstein_order=bo, tight=tight)
+    tb = O.tables(to, dims)
+    ref = O.analyze(to, x.astype(np.float64), tb)
+    xd = from_numpy(x, dev)
+    ws = N.make_workspace(t, xd)
+    c = N.analyze(t, xd, workspace=ws)
+    torch.cuda.synchronize()
+    _check_coeffs(c, ref, np.linalg.norm(x))
+    # synthesis of random coefficients against the oracle's synthesis
+    r = [rng.standard_normal(dims).astype(np.float32) for _ in ref]
+    rd = from_numpy(np.stack(r, axis=-1), dev)
+    y = N.synthesize(t, rd, workspace=ws)
+    yref = O.synthesize(to, [v.astype(np.float64) for v in r], tb)
+    assert _rel(to_numpy(y), yref) <= RTOL
+    # perfect reconstruction
+    xr = N.synthesize(t, c, workspace=ws)
+    assert _rel(to_numpy(xr), x) <= RTOL
+
+
+def test_ns_streaming_matches_materialised_bitwise():
+    dev = _dev()
+    t = N.NSSurfacelet(scales=2, direction_levels=[1, 2], bernstein_order=6)
+    x = from_numpy(np.random.default_rng(92).standard_normal((16, 16, 16)).astype(np.float32), dev)
+    ws = N.make_workspace(t, x)
+    ref = N.analyze(t, x, workspace=ws)
+    seen = []
+
+    def f(i, band):
+        seen.append(i)
+        assert torch.equal(band, N.subband(t, ref, i))
+
+    assert N.analyze(f, t, x, workspace=ws) is None
+    assert seen == list(range(2, N.nsubbands(t, (16, 16, 16)) + 1)) + [1]
+    y1 = N.synthesize(t, ref, workspace=ws)
+    y2 = torch.empty_like(y1)
+    N.synthesize_(lambda i: N.subband(t, ref, i), y2, t, workspace=ws)
+    assert torch.equal(y1, y2)
+    with pytest.raises(surfhip.DimensionMismatch):
+        N.synthesize_(lambda i: torch.zeros(4, 4, 4, device=dev), torch.empty_like(y1), t, workspace=ws)
+
+
+def test_ns_complex_is_two_real_channels_bitwise():
+    dev = _dev()
+    rng = np.random.default_rng(94)
+    xr = rng.standard_normal((16, 16, 16)).astype(np.float32)
+    xi = rng.standard_normal((16, 16, 16)).astype(np.float32)
+    t = N.NSSurfacelet(scales=2, direction_levels=[1, 1], bernstein_order=6)
+    z = from_numpy((xr + 1j * xi).astype(np.complex64), dev)
+    cc = N.analyze(t, z)
+    cr = N.analyze(t, from_numpy(xr, dev))
+    ci = N.analyze(t, from_numpy(xi, dev))
+    torch.cuda.synchronize()
+    assert torch.equal(cc, torch.complex(cr, ci))
+    zr = N.synthesize(t, cc)
+    assert _rel(to_numpy(zr), xr + 1j * xi) <= RTOL
+    yr = N.synthesize(t, cr)
+    yi = N.synthesize(t, ci)
+    assert torch.equal(zr, torch.complex(yr, yi))
+
+
+def test_ns_workspace_reuse_bitwise_and_validated():
+    dev = _dev()
+    t = N.NSSurfacelet(scales=2, direction_levels=[1, 2], bernstein_order=6)
+    x = from_numpy(np.random.default_rng(96).standard_normal((16, 16, 16)).astype(np.float32), dev)
+    ws = N.make_workspace(t, torch.float32, (16, 16, 16))
+    a = N.analyze(t, x)
+    b = N.analyze(t, x, workspace=ws)
+    c = N.analyze(ws, x)
+    assert torch.equal(a, b) and torch.equal(a, c)
+    with pytest.raises(surfhip.DimensionMismatch):
+        N.analyze(t, torch.zeros(8, 8, 8, device=dev), workspace=ws)
+    with pytest.raises(surfhip.ArgumentError, match="element type"):
+        N.analyze(t, torch.complex(x, x), workspace=ws)
+    wsj = N.make_workspace(N.NSSurfacelet(scales=1, direction_levels=[1]), torch.float32, (16, 16, 16))
+    with pytest.raises(surfhip.ArgumentError, match=r"the workspace was built for NSSurfacelet\(scales = 1"):
+        N.analyze(t, x, workspace=wsj)
+    with pytest.raises(surfhip.ArgumentError, match="tasks must be ≥ 1"):
+        N.make_workspace(t, torch.float32, (16, 16, 16), tasks=0)
+
+
+def test_ns_constant_volume_stays_in_lowpass():
+    dev = _dev()
+    t = N.NSSurfacelet(scales=2, direction_levels=[1, 1], bernstein_order=6)
+    c = N.analyze(t, from_numpy(np.full((16, 16, 16), 1.75, np.float32), dev))
+    got = to_numpy(c)
+    assert np.max(np.abs(got[..., 0] - 1.75)) < 1e-5
+    assert np.max(np.abs(got[..., 1:])) < 1e-5
+
+
+def test_ns_shift_equivariance():
+    dev = _dev()
+    t = N.NSSurfacelet(scales=2, direction_levels=[1, 2], bernstein_order=6)
+    x = np.random.default_rng(91).standard_normal((16, 16, 16)).astype(np.float32)
+    s = (3, -2, 5)
+    a = to_numpy(N.analyze(t, from_numpy(x, dev)))
+    b = to_numpy(N.analyze(t, from_numpy(np.roll(x, s, axis=(0, 1, 2)), dev)))
+    assert np.max(np.abs(b - np.roll(a, s, axis=(0, 1, 2)))) < 1e-5
+
+
+def test_ns_128_default_transform_matches_oracle():
+    # The headline configuration at its full size: NSSurfacelet(scales = 3,
+    # direction_levels = [2, 2, 3]) on 128^3 Float32, 49 subbands.
+    dev = _dev()
+    dims = (128, 128, 128)
+    print("case 128^3 default transform", flush=True)
+    x = np.random.default_rng(20260).standard_normal(dims).astype(np.float32)
+    t = N.NSSurfacelet(scales=3, direction_levels=[2, 2, 3])
+    to = O.NSParams(3, (2, 2, 3))
+    tb = O.tables(to, dims)
+    xd = from_numpy(x, dev)
+    ws = N.make_workspace(t, xd)
+    c = N.analyze(t, xd, workspace=ws)
+    torch.cuda.synchronize()
+    ref = O.analyze(to, x.astype(np.float64), tb)
+    _check_coeffs(c, ref, np.linalg.norm(x))
+    del ref
+    # Parseval (tight frame) and perfect reconstruction at full size.
+    cn = to_numpy(c).astype(np.float64)
+    assert abs(np.sum(cn ** 2) / np.sum(x.astype(np.float64) ** 2) - 1) < 1e-5
+    xr = N.synthesize(t, c, workspace=ws)
+    assert _rel(to_numpy(xr), x) <= RTOL
