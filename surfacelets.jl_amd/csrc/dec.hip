@@ -1,0 +1,788 @@
+// Decimated Surfacelet analyze/synthesize on gfx950.
+//
+// Analysis of one real channel (src/transforms/surfacelet.jl:369-381):
+//   X = FFT_3(x) (complex, full grid);
+//   per pyramid level j (finest first): split X into its highpass (in place)
+//   and the resampled lowpass (fourier_pyramid.jl:108-118); then per
+//   hourglass K: S_K = X[half_K] V_K / prod(subband size), the IRC analysis
+//   stages (irc.jl:127-134), inverse FFTs along the two other axes within each
+//   wedge block, and a c2r along K that writes each block's real subband into
+//   the flat coefficient vector (surfacelet_scale.jl:200-213);
+//   coarse subband = real(IFFT_3(lowpass)).
+// Synthesis mirrors it (surfacelet.jl:384-396, surfacelet_scale.jl:220-257).
+// Every index map (pyramid boxes, block ranges, Hermitian mirror, subband
+// offsets) is integer arithmetic fixed on the host and shipped as tables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "dec_tables.hpp"
+#include "fft.hpp"
+#include "hip_util.hpp"
+
+namespace surfhip {
+
+namespace {
+
+constexpr int DT = 256;
+
+__device__ __forceinline__ int64_t lin3(int i1, int i2, int i3, int m1, int m2) {
+  return (int64_t)i1 + (int64_t)m1 * ((int64_t)i2 + (int64_t)m2 * i3);
+}
+
+// Complex FFT along AXIS (1, 2, 3) of an m1 x m2 x m3 column-major array, in
+// segments of length `seg` (seg divides the axis; seg = the axis length for a
+// whole transform).  G pencils per block: consecutive segments for AXIS 1
+// (one contiguous run), consecutive i1 for AXIS 2 and 3.
+template <int AXIS, int DIR>
+__global__ __launch_bounds__(DT) void k_c2c(const float2 *in, float2 *out, int m1, int m2, int m3, int seg, int G,
+                                           FftPlan1D pl, const float2 *__restrict__ twg) {
+  extern __shared__ float2 smem[];
+  float2 *tw = smem;
+  float2 *a = tw + seg;
+  float2 *b = a + (size_t)seg * G;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < seg; k += DT) tw[k] = twg[pl.tw + k];
+  int64_t base0 = 0, sa = 1, sg = 1;
+  int valid = G;
+  if (AXIS == 1) {
+    const int64_t npen = (int64_t)(m1 / seg) * m2 * m3;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    valid = (int)min((int64_t)G, npen - p0);
+    base0 = p0 * seg;
+    sa = 1;
+    sg = seg;
+  } else {
+    const int ub = (m1 + G - 1) / G;
+    const int u = blockIdx.x % ub;
+    const int rest = blockIdx.x / ub;
+    const int u0 = u * G;
+    valid = min(G, m1 - u0);
+    if (AXIS == 2) {
+      const int ns = m2 / seg;
+      const int s = rest % ns, i3 = rest / ns;
+      base0 = lin3(u0, s * seg, i3, m1, m2);
+      sa = m1;
+    } else {
+      const int ns = m3 / seg;
+      const int i2 = rest % m2, s = rest / m2;
+      base0 = lin3(u0, i2, s * seg, m1, m2);
+      sa = (int64_t)m1 * m2;
+    }
+    sg = 1;
+  }
+  for (int idx = tid; idx < seg * G; idx += DT) {
+    int t, g;
+    if (AXIS == 1) {
+      g = idx / seg;
+      t = idx - g * seg;
+    } else {
+      t = idx / G;
+      g = idx - t * G;
+    }
+    a[t * G + g] = g < valid ? in[base0 + g * sg + t * sa] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  float2 *res = lds_fft<DIR>(a, b, pl, G, tw, tid, DT);
+  for (int idx = tid; idx < seg * G; idx += DT) {
+    int t, g;
+    if (AXIS == 1) {
+      g = idx / seg;
+      t = idx - g * seg;
+    } else {
+      t = idx / G;
+      g = idx - t * G;
+    }
+    if (g < valid) out[base0 + g * sg + t * sa] = res[t * G + g];
+  }
+}
+
+__global__ void k_dzero(float2 *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_float2(0.0f, 0.0f);
+}
+
+// X = complex(x), x read with element stride `is`.
+__global__ void k_to_complex(const float *__restrict__ x, int64_t is, float2 *__restrict__ X, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    X[i] = make_float2(x[i * is], 0.0f);
+}
+
+// out[i * os] = Re(X[i]) * scale.
+__global__ void k_real_out(const float2 *__restrict__ X, float *__restrict__ out, int64_t os, float scale, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i * os] = X[i].x * scale;
+}
+
+struct PyrDev {
+  int n1, n2, n3, l1, l2, l3;        // spectrum and lowpass sizes
+  int len1, len2, len3;              // box extents (2P + 1)
+  const int *idx, *idx_low;          // concatenated per axis
+  const float *prof, *comp;          // profile and 1 - profile, per axis
+  float D15;
+};
+
+__device__ __forceinline__ void pyr_factors(const PyrDev &P, int a, int b, int c, int64_t &xi, int64_t &li, float &L,
+                                            float &s) {
+  const int ob = P.len1, oc = P.len1 + P.len2;
+  xi = lin3(P.idx[a], P.idx[ob + b], P.idx[oc + c], P.n1, P.n2);
+  li = lin3(P.idx_low[a], P.idx_low[ob + b], P.idx_low[oc + c], P.l1, P.l2);
+  const float p1 = P.prof[a], p2 = P.prof[ob + b], p3 = P.prof[oc + c];
+  L = (p1 * p2) * p3;
+  // 1 - L from the FP64 complements, then 1 - L^2 = (1 - L)(1 + L).
+  const float om = P.comp[a] + p1 * P.comp[ob + b] + (p1 * p2) * P.comp[oc + c];
+  const float r = om * (1.0f + L);
+  s = sqrtf(r > 0.0f ? r : 0.0f);
+}
+
+// low[idx_low] = L X[idx] / D^1.5; X[idx] *= sqrt(1 - L^2).
+__global__ void k_pyr_split(float2 *__restrict__ X, float2 *__restrict__ low, PyrDev P) {
+  const int64_t n = (int64_t)P.len1 * P.len2 * P.len3;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int a = e % P.len1;
+    const int64_t r = e / P.len1;
+    const int b = r % P.len2, c = r / P.len2;
+    int64_t xi, li;
+    float L, s;
+    pyr_factors(P, a, b, c, xi, li, L, s);
+    const float2 v = X[xi];
+    const float f = L / P.D15;
+    low[li] = make_float2(v.x * f, v.y * f);
+    X[xi] = make_float2(v.x * s, v.y * s);
+  }
+}
+
+// X[idx] = sqrt(1 - L^2) X[idx] + L D^1.5 low[idx_low].
+__global__ void k_pyr_merge(float2 *__restrict__ X, const float2 *__restrict__ low, PyrDev P) {
+  const int64_t n = (int64_t)P.len1 * P.len2 * P.len3;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int a = e % P.len1;
+    const int64_t r = e / P.len1;
+    const int b = r % P.len2, c = r / P.len2;
+    int64_t xi, li;
+    float L, s;
+    pyr_factors(P, a, b, c, xi, li, L, s);
+    const float2 v = X[xi], w = low[li];
+    const float f = L * P.D15;
+    X[xi] = make_float2(s * v.x + f * w.x, s * v.y + f * w.y);
+  }
+}
+
+// S = X[half] . V . scale over the half-spectrum hd of axis K.
+__global__ void k_masked(const float2 *__restrict__ X, int n1, int n2, float2 *__restrict__ S, const float *__restrict__ V,
+                         int h1, int h2, int h3, float scale) {
+  const int64_t n = (int64_t)h1 * h2 * h3;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i1 = e % h1;
+    const int64_t r = e / h1;
+    const int i2 = r % h2, i3 = r / h2;
+    const float2 v = X[lin3(i1, i2, i3, n1, n2)];
+    const float m = V[e] * scale;
+    S[e] = make_float2(v.x * m, v.y * m);
+  }
+}
+
+// One IRC stage on S (dims h, hourglass axis K, decimated axis m, block b):
+// analysis A <- F0(A) A + F0(B) B, B <- F1(A) A + F1(B) B; synthesis
+// A <- G0(A) A + G1(A) B, B <- G0(B) A + G1(B) B (irc.jl:105-146).
+__global__ void k_irc(float2 *__restrict__ S, int h1, int h2, int h3, int K, int m, int b,
+                      const float2 *__restrict__ f0, const float2 *__restrict__ f1, int synthesis) {
+  const int hd[3] = {h1, h2, h3};
+  const int nm = hd[m], nKh = hd[K];
+  const int64_t npairs = (int64_t)h1 * h2 * h3 / 2;
+  const int64_t stride_m = m == 0 ? 1 : (m == 1 ? (int64_t)h1 : (int64_t)h1 * h2);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += (int64_t)gridDim.x * blockDim.x) {
+    // q enumerates (index along m restricted to A halves, other indices).
+    int idx[3];
+    int64_t r = q;
+    for (int d = 0; d < 3; ++d) {
+      const int ext = d == m ? nm / 2 : hd[d];
+      idx[d] = r % ext;
+      r /= ext;
+    }
+    const int pi = idx[m];
+    const int cA = (pi / b) * 2 * b + (pi % b), cB = cA + b;
+    idx[m] = cA;
+    const int64_t eA = lin3(idx[0], idx[1], idx[2], h1, h2);
+    const int64_t eB = eA + (int64_t)b * stride_m;
+    const int kK = idx[K];
+    const float2 A = S[eA], B = S[eB];
+    const float2 f0A = f0[kK + (int64_t)nKh * cA], f0B = f0[kK + (int64_t)nKh * cB];
+    const float2 f1A = f1[kK + (int64_t)nKh * cA], f1B = f1[kK + (int64_t)nKh * cB];
+    float2 nA, nB;
+    if (!synthesis) {
+      nA = cadd(cmul(f0A, A), cmul(f0B, B));
+      nB = cadd(cmul(f1A, A), cmul(f1B, B));
+    } else {
+      nA = cadd(cmul(f0A, A), cmul(f1A, B));
+      nB = cadd(cmul(f0B, A), cmul(f1B, B));
+    }
+    S[eA] = nA;
+    S[eB] = nB;
+  }
+}
+
+// Where the real subbands of hourglass K of one scale live: the element of
+// line (o-indices) at position t along K.
+struct BlockMap {
+  int K;
+  int h[3];          // half-spectrum dims (h[K] = nK/2+1)
+  int nK;
+  int bs[3];         // block size along the other axes (bs[K] unused)
+  int cnt[3];        // block counts (cnt[K] = 1)
+  int sb[3];         // subband shape: sb[K] = nK, bs elsewhere
+  const int64_t *off;  // element offset of each block's subband
+};
+
+__device__ __forceinline__ int64_t block_elem(const BlockMap &M, int o[3], int t) {
+  int q[3], l[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d == M.K) {
+      q[d] = 0;
+      l[d] = t;
+    } else {
+      q[d] = o[d] / M.bs[d];
+      l[d] = o[d] - q[d] * M.bs[d];
+    }
+  }
+  const int i = q[0] + M.cnt[0] * (q[1] + M.cnt[1] * q[2]);
+  return M.off[i] + lin3(l[0], l[1], l[2], M.sb[0], M.sb[1]);
+}
+
+// Lines along K: for K = 0 consecutive lines are consecutive (i2, i3) rows;
+// otherwise consecutive i1 for fixed remaining index.
+__device__ __forceinline__ void line_index(const BlockMap &M, int64_t line, int o[3]) {
+  const int K = M.K;
+  if (K == 0) {
+    o[0] = 0;
+    o[1] = line % M.h[1];
+    o[2] = line / M.h[1];
+  } else if (K == 1) {
+    o[1] = 0;
+    o[0] = line % M.h[0];
+    o[2] = line / M.h[0];
+  } else {
+    o[2] = 0;
+    o[0] = line % M.h[0];
+    o[1] = line / M.h[0];
+  }
+}
+
+__device__ __forceinline__ int64_t half_elem(const BlockMap &M, const int o[3], int t) {
+  int i[3] = {o[0], o[1], o[2]};
+  i[M.K] = t;
+  return lin3(i[0], i[1], i[2], M.h[0], M.h[1]);
+}
+
+// Subband lines = c2r along K of S (unnormalised), scattered into the
+// coefficients (element stride os).  G lines per block.
+__global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, float *__restrict__ out, int64_t os,
+                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+  extern __shared__ float2 smem[];
+  const int nK = M.nK, hK = M.h[M.K];
+  float2 *tw = smem;
+  float2 *st = tw + nK;
+  float2 *a = st + (size_t)hK * G;
+  float2 *b = a + (size_t)nK * G;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < nK; k += DT) tw[k] = twg[pl.tw + k];
+  const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
+  const int64_t l0 = (int64_t)blockIdx.x * G;
+  const int valid = (int)min((int64_t)G, nlines - l0);
+  for (int idx = tid; idx < hK * G; idx += DT) {
+    int t, g;
+    if (M.K == 0) {
+      g = idx / hK;
+      t = idx - g * hK;
+    } else {
+      t = idx / G;
+      g = idx - t * G;
+    }
+    float2 v = make_float2(0.0f, 0.0f);
+    if (g < valid) {
+      int o[3];
+      line_index(M, l0 + g, o);
+      v = S[half_elem(M, o, t)];
+    }
+    st[t * G + g] = v;
+  }
+  __syncthreads();
+  const bool even = (nK & 1) == 0;
+  for (int idx = tid; idx < nK * G; idx += DT) {
+    const int t = idx / G, g = idx - t * G;
+    float2 z;
+    if (t < hK) {
+      z = st[t * G + g];
+      if (t == 0 || (even && t == nK / 2)) z.y = 0.0f;
+    } else {
+      z = conjf2(st[(nK - t) * G + g]);
+    }
+    a[idx] = z;
+  }
+  __syncthreads();
+  float2 *res = lds_fft<1>(a, b, pl, G, tw, tid, DT);
+  for (int idx = tid; idx < nK * G; idx += DT) {
+    int t, g;
+    if (M.K == 0) {
+      g = idx / nK;
+      t = idx - g * nK;
+    } else {
+      t = idx / G;
+      g = idx - t * G;
+    }
+    if (g >= valid) continue;
+    int o[3];
+    line_index(M, l0 + g, o);
+    out[block_elem(M, o, t) * os] = res[t * G + g].x;
+  }
+}
+
+// S lines = r2c along K of the subbands' lines (the first nK/2+1 bins).
+__global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, int64_t is, float2 *__restrict__ S,
+                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+  extern __shared__ float2 smem[];
+  const int nK = M.nK, hK = M.h[M.K];
+  float2 *tw = smem;
+  float2 *a = tw + nK;
+  float2 *b = a + (size_t)nK * G;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < nK; k += DT) tw[k] = twg[pl.tw + k];
+  const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
+  const int64_t l0 = (int64_t)blockIdx.x * G;
+  const int valid = (int)min((int64_t)G, nlines - l0);
+  for (int idx = tid; idx < nK * G; idx += DT) {
+    int t, g;
+    if (M.K == 0) {
+      g = idx / nK;
+      t = idx - g * nK;
+    } else {
+      t = idx / G;
+      g = idx - t * G;
+    }
+    float v = 0.0f;
+    if (g < valid) {
+      int o[3];
+      line_index(M, l0 + g, o);
+      v = in[block_elem(M, o, t) * is];
+    }
+    a[t * G + g] = make_float2(v, 0.0f);
+  }
+  __syncthreads();
+  float2 *res = lds_fft<-1>(a, b, pl, G, tw, tid, DT);
+  for (int idx = tid; idx < hK * G; idx += DT) {
+    int t, g;
+    if (M.K == 0) {
+      g = idx / hK;
+      t = idx - g * hK;
+    } else {
+      t = idx / G;
+      g = idx - t * G;
+    }
+    if (g >= valid) continue;
+    int o[3];
+    line_index(M, l0 + g, o);
+    S[half_elem(M, o, t)] = res[t * G + g];
+  }
+}
+
+// X += c M S on the half of axis K and X[-w] += c M conj(S) on its strictly
+// interior mirror, index i -> (n - i) mod n (surfacelet_scale.jl:246-257).
+__global__ void k_hermitian_add(float2 *__restrict__ X, int n1, int n2, int n3, const float2 *__restrict__ S,
+                                const float *__restrict__ M, int K, int h1, int h2, int h3, float c, int mirror) {
+  const int n[3] = {n1, n2, n3};
+  const int64_t tot = (int64_t)h1 * h2 * h3;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    int i[3];
+    i[0] = e % h1;
+    const int64_t r = e / h1;
+    i[1] = r % h2;
+    i[2] = r / h2;
+    const float2 s = S[e];
+    const float w = c * M[e];
+    if (!mirror) {
+      const int64_t xi = lin3(i[0], i[1], i[2], n1, n2);
+      const float2 x = X[xi];
+      X[xi] = make_float2(x.x + w * s.x, x.y + w * s.y);
+    } else {
+      const int iK = i[K];
+      if (iK < 1 || iK > n[K] / 2 - 1) continue;
+      int j[3];
+      for (int d = 0; d < 3; ++d) j[d] = d == K ? n[K] - iK : (n[d] - i[d]) % n[d];
+      const int64_t xj = lin3(j[0], j[1], j[2], n1, n2);
+      const float2 x = X[xj];
+      X[xj] = make_float2(x.x + w * s.x, x.y - w * s.y);
+    }
+  }
+}
+
+int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + DT - 1) / DT, 4096)); }
+
+}  // namespace
+
+}  // namespace surfhip
+
+using namespace surfhip;
+
+struct DecLevelDev {
+  PyrDev pd;
+  DevBuf<int> idx, idx_low;
+  DevBuf<float> prof, comp;
+};
+
+struct DecScaleDev {
+  std::array<DevBuf<int64_t>, 3> boff;  // per K: subband offset of each block
+  std::array<int, 3> first;            // 0-based subband index of each K's first block
+};
+
+struct surfhip_s_plan {
+  DecTables T;
+  int dtype = SURFHIP_F32, nch = 1;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<DecLevelDev>> lev;
+  std::vector<std::unique_ptr<DecScaleDev>> sc;
+  DevBuf<float> masks;
+  DevBuf<float2> filt, tw;
+  std::vector<DevBuf<float2>> spectra;  // [j] the spectrum entering level j; [J] the coarse lowpass
+  DevBuf<float2> Sbuf[3];
+  std::vector<std::pair<int, FftPlan1D>> plans;
+
+  const FftPlan1D &plan_for(int n) const {
+    for (auto &p : plans)
+      if (p.first == n) return p.second;
+    fail(SURFHIP_ERR_DEVICE, fmt("no FFT plan for length %d", n));
+  }
+
+  void launch_c2c(int axis, int dir, float2 *A, Dims3 m, int seg) {
+    const FftPlan1D &pl = plan_for(seg);
+    int G = std::max(1, std::min(16, 4096 / seg));
+    const size_t lds = sizeof(float2) * ((size_t)seg + 2 * (size_t)seg * G);
+    if (lds > 65536) fail(SURFHIP_ERR_ARG, fmt("axis length %d is too long for the decimated path", seg));
+    int64_t blocks;
+    if (axis == 0) {
+      const int64_t npen = (int64_t)(m[0] / seg) * m[1] * m[2];
+      blocks = (npen + G - 1) / G;
+    } else {
+      const int64_t ub = (m[0] + G - 1) / G;
+      blocks = axis == 1 ? ub * (m[1] / seg) * m[2] : ub * m[1] * (m[2] / seg);
+    }
+#define SH_C2C(AX, D) \
+  hipLaunchKernelGGL((k_c2c<AX, D>), dim3((unsigned)blocks), dim3(DT), lds, stream, A, A, m[0], m[1], m[2], seg, G, pl, tw.p)
+    if (axis == 0) { if (dir < 0) SH_C2C(1, -1); else SH_C2C(1, 1); }
+    else if (axis == 1) { if (dir < 0) SH_C2C(2, -1); else SH_C2C(2, 1); }
+    else { if (dir < 0) SH_C2C(3, -1); else SH_C2C(3, 1); }
+#undef SH_C2C
+    SH_LAUNCHED();
+  }
+  void fft3(int dir, float2 *A, Dims3 m) {
+    for (int ax = 0; ax < 3; ++ax) launch_c2c(ax, dir, A, m, m[ax]);
+  }
+  void zero(float2 *p, int64_t n) {
+    hipLaunchKernelGGL(k_dzero, dim3(grid_for(n)), dim3(DT), 0, stream, p, n);
+    SH_LAUNCHED();
+  }
+  BlockMap block_map(int j, int K) const {
+    const DecScaleT &s = T.scales[j];
+    Dims3 counts, bsize;
+    scale_blocks(s, K, counts, bsize);
+    Dims3 hd = half_dims(s.n, K);
+    Dims3 sb = subband_size(s, K);
+    BlockMap M;
+    M.K = K;
+    M.nK = s.n[K];
+    for (int d = 0; d < 3; ++d) {
+      M.h[d] = hd[d];
+      M.bs[d] = bsize[d];
+      M.cnt[d] = d == K ? 1 : counts[d];
+      M.sb[d] = sb[d];
+    }
+    M.off = sc[j]->boff[K].p;
+    return M;
+  }
+  void lines(bool c2r, int j, int K, float2 *S, float *coef, int64_t stride) {
+    BlockMap M = block_map(j, K);
+    const int nK = M.nK, hK = M.h[K];
+    const FftPlan1D &pl = plan_for(nK);
+    int G = std::max(1, std::min(16, 2048 / nK));
+    const size_t lds = sizeof(float2) * ((size_t)nK + (size_t)hK * G + 2 * (size_t)nK * G);
+    if (lds > 65536) fail(SURFHIP_ERR_ARG, fmt("axis length %d is too long for the decimated path", nK));
+    const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
+    const unsigned blocks = (unsigned)((nlines + G - 1) / G);
+    if (c2r)
+      hipLaunchKernelGGL(k_c2r_lines, dim3(blocks), dim3(DT), lds, stream, S, coef, stride, M, G, pl, tw.p);
+    else
+      hipLaunchKernelGGL(k_r2c_lines, dim3(blocks), dim3(DT), lds, stream, (const float *)coef, stride, S, M, G, pl,
+                         tw.p);
+    SH_LAUNCHED();
+  }
+
+  void analyze_channel(const float *x, float *coef) {
+    const int64_t os = nch;
+    Dims3 n = T.dims;
+    const int64_t N = (int64_t)n[0] * n[1] * n[2];
+    float2 *X = spectra[0].p;
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, stream, x, (int64_t)nch, X, N);
+    SH_LAUNCHED();
+    fft3(-1, X, n);
+    for (int j = 0; j < T.J; ++j) {
+      const PyrLevel &pl = T.pyr[j];
+      float2 *low = spectra[j + 1].p;
+      zero(low, (int64_t)pl.nlow[0] * pl.nlow[1] * pl.nlow[2]);
+      const PyrDev &pd = lev[j]->pd;
+      hipLaunchKernelGGL(k_pyr_split, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, stream,
+                         spectra[j].p, low, pd);
+      SH_LAUNCHED();
+      const DecScaleT &s = T.scales[j];
+      for (int K = 0; K < 3; ++K) {
+        Dims3 hd = half_dims(s.n, K);
+        Dims3 sb = subband_size(s, K);
+        const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
+        float2 *S = Sbuf[K].p;
+        const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
+        hipLaunchKernelGGL(k_masked, dim3(grid_for(HN)), dim3(DT), 0, stream, spectra[j].p, s.n[0], s.n[1], S,
+                           masks.p + s.mask_off[K], hd[0], hd[1], hd[2], scale);
+        SH_LAUNCHED();
+        for (const IRCStageT &st : s.stages[K]) {
+          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, stream, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+                             filt.p + st.off[0], filt.p + st.off[1], 0);
+          SH_LAUNCHED();
+        }
+        Dims3 counts, bsize;
+        scale_blocks(s, K, counts, bsize);
+        for (int d = 0; d < 3; ++d)
+          if (d != K) launch_c2c(d, 1, S, hd, bsize[d]);
+        lines(true, j, K, S, coef, os);
+      }
+    }
+    const Dims3 nl = T.pyr.back().nlow;
+    const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
+    float2 *C = spectra[T.J].p;
+    fft3(1, C, nl);
+    hipLaunchKernelGGL(k_real_out, dim3(grid_for(NL)), dim3(DT), 0, stream, C, coef + T.offsets[0] * nch, os,
+                       (float)(1.0 / (double)NL), NL);
+    SH_LAUNCHED();
+  }
+
+  void synthesize_channel(const float *coef, float *x) {
+    const int64_t is = nch;
+    const Dims3 nl = T.pyr.back().nlow;
+    const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
+    float2 *C = spectra[T.J].p;
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, stream, coef + T.offsets[0] * nch, is, C, NL);
+    SH_LAUNCHED();
+    fft3(-1, C, nl);
+    for (int j = T.J - 1; j >= 0; --j) {
+      const DecScaleT &s = T.scales[j];
+      float2 *X = spectra[j].p;
+      const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
+      for (int K = 0; K < 3; ++K) {
+        Dims3 hd = half_dims(s.n, K);
+        float2 *S = Sbuf[K].p;
+        lines(false, j, K, S, const_cast<float *>(coef), is);
+        Dims3 counts, bsize;
+        scale_blocks(s, K, counts, bsize);
+        for (int d = 0; d < 3; ++d)
+          if (d != K) launch_c2c(d, -1, S, hd, bsize[d]);
+        const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
+        for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
+          const IRCStageT &st = s.stages[K][q];
+          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, stream, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+                             filt.p + st.off[2], filt.p + st.off[3], 1);
+          SH_LAUNCHED();
+        }
+      }
+      zero(X, NX);
+      for (int K = 0; K < 3; ++K) {
+        Dims3 hd = half_dims(s.n, K);
+        const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
+        for (int mirror = 0; mirror < 2; ++mirror) {
+          hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, stream, X, s.n[0], s.n[1], s.n[2],
+                             Sbuf[K].p, masks.p + s.mask_off[K], K, hd[0], hd[1], hd[2], (float)(1.0 / 3.0), mirror);
+          SH_LAUNCHED();
+        }
+      }
+      const PyrDev &pd = lev[j]->pd;
+      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, stream, X,
+                         spectra[j + 1].p, pd);
+      SH_LAUNCHED();
+    }
+    const Dims3 n = T.dims;
+    const int64_t N = (int64_t)n[0] * n[1] * n[2];
+    fft3(1, spectra[0].p, n);
+    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, stream, spectra[0].p, x, (int64_t)nch,
+                       (float)(1.0 / (double)N), N);
+    SH_LAUNCHED();
+  }
+};
+
+namespace {
+
+surfhip_s_params checked(const surfhip_s_params *p) {
+  if (!p) fail(SURFHIP_ERR_ARG, "null argument");
+  dec_check_params(*p);
+  return *p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int surfhip_s_check_size(const surfhip_s_params *p, int32_t n1, int32_t n2, int32_t n3) {
+  return guarded([&] { dec_check_size(checked(p), Dims3{n1, n2, n3}); });
+}
+
+int surfhip_s_counts(const surfhip_s_params *p, int32_t n1, int32_t n2, int32_t n3, int32_t *nsub, int64_t *ncoef) {
+  return guarded([&] {
+    auto sizes = dec_layout_sizes(checked(p), Dims3{n1, n2, n3});
+    int64_t tot = 0;
+    for (auto &s : sizes) tot += (int64_t)s[0] * s[1] * s[2];
+    if (nsub) *nsub = (int32_t)sizes.size();
+    if (ncoef) *ncoef = tot;
+  });
+}
+
+int surfhip_s_layout(const surfhip_s_params *p, int32_t n1, int32_t n2, int32_t n3, int64_t *sizes3, int64_t *offsets,
+                     int32_t cap) {
+  return guarded([&] {
+    auto sizes = dec_layout_sizes(checked(p), Dims3{n1, n2, n3});
+    if ((int)sizes.size() > cap) fail(SURFHIP_ERR_DIM, fmt("layout has %zu subbands, buffer holds %d", sizes.size(), cap));
+    int64_t off = 0;
+    for (size_t i = 0; i < sizes.size(); ++i) {
+      for (int d = 0; d < 3; ++d)
+        if (sizes3) sizes3[3 * i + d] = sizes[i][d];
+      if (offsets) offsets[i] = off;
+      off += (int64_t)sizes[i][0] * sizes[i][1] * sizes[i][2];
+    }
+  });
+}
+
+int surfhip_s_plan_create(surfhip_s_plan **out, const surfhip_s_params *p, int32_t n1, int32_t n2, int32_t n3,
+                          int32_t dtype, void *stream) {
+  return guarded([&] {
+    if (!out) fail(SURFHIP_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (dtype != SURFHIP_F32 && dtype != SURFHIP_C64) fail(SURFHIP_ERR_ARG, fmt("unknown dtype %d", dtype));
+    auto P = std::make_unique<surfhip_s_plan>();
+    dec_build_tables(checked(p), Dims3{n1, n2, n3}, P->T);
+    P->dtype = dtype;
+    P->nch = dtype == SURFHIP_C64 ? 2 : 1;
+    P->stream = (hipStream_t)stream;
+    const DecTables &T = P->T;
+    // FFT plans of every length used: level sizes, block sizes, coarse size.
+    std::vector<int> lens;
+    auto add = [&](int n) {
+      if (std::find(lens.begin(), lens.end(), n) == lens.end()) lens.push_back(n);
+    };
+    std::vector<float2> pool;
+    for (int d = 0; d < 3; ++d) add(T.pyr.back().nlow[d]);
+    for (const DecScaleT &s : T.scales)
+      for (int K = 0; K < 3; ++K) {
+        Dims3 c, b;
+        scale_blocks(s, K, c, b);
+        add(s.n[K]);
+        for (int d = 0; d < 3; ++d) {
+          add(s.n[d]);
+          if (d != K) add(b[d]);
+        }
+      }
+    for (int n : lens) P->plans.push_back({n, make_fft_plan(n, pool)});
+    P->tw.upload(pool);
+    P->masks.upload(T.masks);
+    if (T.filt.empty()) P->T.filt.push_back(make_float2(0, 0));
+    P->filt.upload(P->T.filt);
+    // Pyramid tables.
+    for (const PyrLevel &pl : T.pyr) {
+      auto L = std::make_unique<DecLevelDev>();
+      std::vector<int> ix, il;
+      std::vector<float> pr, cm;
+      for (int a = 0; a < 3; ++a) {
+        ix.insert(ix.end(), pl.idx[a].begin(), pl.idx[a].end());
+        il.insert(il.end(), pl.idx_low[a].begin(), pl.idx_low[a].end());
+        for (double v : pl.prof[a]) {
+          pr.push_back((float)v);
+          cm.push_back((float)(1.0 - v));
+        }
+      }
+      L->idx.upload(ix);
+      L->idx_low.upload(il);
+      L->prof.upload(pr);
+      L->comp.upload(cm);
+      PyrDev &pd = L->pd;
+      pd.n1 = pl.n[0];
+      pd.n2 = pl.n[1];
+      pd.n3 = pl.n[2];
+      pd.l1 = pl.nlow[0];
+      pd.l2 = pl.nlow[1];
+      pd.l3 = pl.nlow[2];
+      pd.len1 = (int)pl.idx[0].size();
+      pd.len2 = (int)pl.idx[1].size();
+      pd.len3 = (int)pl.idx[2].size();
+      pd.idx = L->idx.p;
+      pd.idx_low = L->idx_low.p;
+      pd.prof = L->prof.p;
+      pd.comp = L->comp.p;
+      pd.D15 = (float)pl.D15;
+      P->lev.push_back(std::move(L));
+    }
+    // Block offsets per scale and hourglass.
+    size_t smax[3] = {0, 0, 0};
+    for (size_t j = 0; j < T.scales.size(); ++j) {
+      const DecScaleT &s = T.scales[j];
+      auto D = std::make_unique<DecScaleDev>();
+      int sub = s.first_subband;
+      for (int K = 0; K < 3; ++K) {
+        Dims3 c, b;
+        scale_blocks(s, K, c, b);
+        const int nb = c[0] * c[1] * c[2];
+        std::vector<int64_t> off(nb);
+        for (int i = 0; i < nb; ++i) off[i] = T.offsets[sub + i];
+        D->first[K] = sub;
+        D->boff[K].upload(off);
+        sub += nb;
+        Dims3 hd = half_dims(s.n, K);
+        smax[K] = std::max(smax[K], (size_t)hd[0] * hd[1] * hd[2]);
+      }
+      P->sc.push_back(std::move(D));
+    }
+    for (int K = 0; K < 3; ++K) P->Sbuf[K].alloc(smax[K]);
+    P->spectra.resize(T.J + 1);
+    for (int j = 0; j <= T.J; ++j) {
+      Dims3 n = j < T.J ? T.pyr[j].n : T.pyr.back().nlow;
+      P->spectra[j].alloc((size_t)n[0] * n[1] * n[2]);
+    }
+    SH_HIP(hipDeviceSynchronize());
+    *out = P.release();
+  });
+}
+
+int surfhip_s_plan_destroy(surfhip_s_plan *plan) {
+  return guarded([&] {
+    if (!plan) return;
+    (void)hipStreamSynchronize(plan->stream);
+    delete plan;
+  });
+}
+
+int surfhip_s_set_stream(surfhip_s_plan *plan, void *stream) {
+  return guarded([&] {
+    if (!plan) fail(SURFHIP_ERR_ARG, "null plan");
+    plan->stream = (hipStream_t)stream;
+  });
+}
+
+int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs) {
+  return guarded([&] {
+    if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    for (int ch = 0; ch < plan->nch; ++ch) plan->analyze_channel((const float *)x + ch, (float *)coeffs + ch);
+  });
+}
+
+int surfhip_s_synthesize(surfhip_s_plan *plan, const void *coeffs, void *x) {
+  return guarded([&] {
+    if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    for (int ch = 0; ch < plan->nch; ++ch) plan->synthesize_channel((const float *)coeffs + ch, (float *)x + ch);
+  });
+}
+
+}  // extern "C"

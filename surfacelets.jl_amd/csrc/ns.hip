@@ -15,11 +15,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
 #include "fft.hpp"
 #include "hip_util.hpp"
+#include "ns_fast.hpp"
 #include "ns_tables.hpp"
 
 namespace surfhip {
@@ -182,6 +184,9 @@ __global__ void k_zero(float2 *__restrict__ p, int64_t n) {
 
 using namespace surfhip;
 
+// A plan runs one of two kernel sets on the same tables: the fast fused path
+// (ns_fast.hip) on power-of-two grids it covers, or the general three-pass
+// path above on any other size.  Both evaluate the same mask expression.
 struct surfhip_ns_plan {
   NSTables T;
   int dtype = SURFHIP_F32;
@@ -193,8 +198,22 @@ struct surfhip_ns_plan {
   int g_rows = 16, g_axis = 16;
   DevBuf<float> hg1, hg1c, hg2, hg2c, a, l, c, planes;
   DevBuf<float2> tw, X[2], acc[2], work;
+  DevBuf<NSItem> ditems;
   NSMaskView dview{};
   bool streaming_fwd = false;
+  // fast path
+  bool fast = false;
+  FastOps ops;
+  int maxw = 1;                  // largest group
+  DevBuf<float2> VT, Vfin, accg[2];
+  int open_group = -2;           // streaming synthesis: group being accumulated (-2: none)
+  int open_item = 0;             // an item of that group
+  bool open_fresh = true;        // no term accumulated yet in the open group
+
+  FastCtx ctx() const {
+    return FastCtx{stream, n1, n2, n3, h1, H, dview, ditems.p, tw.p + f1.tw, tw.p + f2.tw, tw.p + f3.tw};
+  }
+  int group_of(int i) const { return T.items[i].lowpass ? -1 : T.items[i].group; }
 
   size_t lds_rows_r2c() const { return sizeof(float2) * ((size_t)n1 + 2 * (size_t)n1 * (g_rows / 2)); }
   size_t lds_rows_c2r() const {
@@ -233,24 +252,98 @@ struct surfhip_ns_plan {
   void forward_spectrum(const float *x) {
     static const NSItem none{};
     for (int ch = 0; ch < nch; ++ch) {
+      if (fast) {
+        const FastCtx c = ctx();
+        ops.b_fwd(c, x + ch, 0, nch, 1, VT.p);
+        ops.a_fwd(c, VT.p, X[ch].p);
+        continue;
+      }
       r2c(x + ch, nch, X[ch].p);
       axis<2, -1, 0>(X[ch].p, X[ch].p, none);
       axis<3, -1, 0>(X[ch].p, X[ch].p, none);
     }
   }
-  // band = irfft(X . M_i)
-  void subband(int i, float *band) {
+  // Subbands item0 .. item0+nw-1 (one group) into out (subband stride N nch).
+  void subbands(int item0, int nw, float *out) {
     static const NSItem none{};
-    const NSItem &it = T.items[i];
+    const float scale = 1.0f / (float)N;
     for (int ch = 0; ch < nch; ++ch) {
-      axis<3, 1, 1>(X[ch].p, work.p, it);
-      axis<2, 1, 0>(work.p, work.p, none);
-      c2r(work.p, band + ch, nch, 1.0f / (float)N);
+      if (fast) {
+        const FastCtx c = ctx();
+        ops.a_ana(c, X[ch].p, VT.p, item0, nw);
+        ops.b_inv(c, VT.p, nw, out + ch, (size_t)N * nch, (size_t)nch, scale);
+        continue;
+      }
+      for (int w = 0; w < nw; ++w) {
+        axis<3, 1, 1>(X[ch].p, work.p, T.items[item0 + w]);
+        axis<2, 1, 0>(work.p, work.p, none);
+        c2r(work.p, out + (int64_t)w * N * nch + ch, nch, scale);
+      }
     }
+  }
+  void analyze(const float *x, float *coeffs) {
+    forward_spectrum(x);
+    for (size_t g = 0; g < T.group_first.size(); ++g) {
+      const int i0 = T.group_first[g], nw = T.group_count[g];
+      subbands(i0, nw, coeffs + (int64_t)i0 * N * nch);
+    }
+    subbands(0, 1, coeffs);
+  }
+  void synthesize(const float *coeffs, float *x) {
+    if (!fast) {
+      inverse_begin();
+      for (int i = 1; i < T.S; ++i) accumulate(i, coeffs + (int64_t)i * N * nch);
+      accumulate(0, coeffs);
+      finish(x);
+      return;
+    }
+    const FastCtx c = ctx();
+    for (int ch = 0; ch < nch; ++ch) zero(acc[ch].p);
+    const int G = (int)T.group_first.size();
+    for (int g = 0; g <= G; ++g) {
+      const int i0 = g < G ? T.group_first[g] : 0, nw = g < G ? T.group_count[g] : 1;
+      const bool last = g == G;
+      for (int ch = 0; ch < nch; ++ch) {
+        ops.b_fwd(c, coeffs + (int64_t)i0 * N * nch + ch, (size_t)N * nch, (size_t)nch, nw, VT.p);
+        ops.a_syn(c, VT.p, nw, acc[ch].p, nullptr, nullptr, Vfin.p, i0, 1, last ? 1 : 0);
+        if (last) ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N);
+      }
+    }
+    open_group = -2;
+  }
+  void inverse_begin() {
+    for (int ch = 0; ch < nch; ++ch) zero(acc[ch].p);
+    open_group = -2;
+    open_fresh = true;
+  }
+  void close_open() {
+    if (open_group == -2) return;
+    if (!open_fresh) {
+      const FastCtx c = ctx();
+      for (int ch = 0; ch < nch; ++ch) ops.a_close(c, acc[ch].p, accg[ch].p, open_item);
+    }
+    open_group = -2;
+    open_fresh = true;
   }
   // acc += rfft(band) . M_i
   void accumulate(int i, const float *band) {
     static const NSItem none{};
+    if (fast) {
+      const int g = group_of(i);
+      if (g != open_group) {
+        close_open();
+        open_group = g;
+        open_item = i;
+        open_fresh = true;
+      }
+      const FastCtx c = ctx();
+      for (int ch = 0; ch < nch; ++ch) {
+        ops.b_fwd(c, band + ch, 0, (size_t)nch, 1, VT.p);
+        ops.a_syn(c, VT.p, 1, acc[ch].p, open_fresh ? nullptr : accg[ch].p, accg[ch].p, nullptr, i, 0, 0);
+      }
+      open_fresh = false;
+      return;
+    }
     const NSItem &it = T.items[i];
     for (int ch = 0; ch < nch; ++ch) {
       r2c(band + ch, nch, work.p);
@@ -260,14 +353,20 @@ struct surfhip_ns_plan {
   }
   void finish(float *x) {
     static const NSItem none{};
+    if (fast) {
+      close_open();
+      const FastCtx c = ctx();
+      for (int ch = 0; ch < nch; ++ch) {
+        ops.a_inv(c, acc[ch].p, Vfin.p);
+        ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N);
+      }
+      return;
+    }
     for (int ch = 0; ch < nch; ++ch) {
       axis<3, 1, 0>(acc[ch].p, work.p, none);
       axis<2, 1, 0>(work.p, work.p, none);
       c2r(work.p, x + ch, nch, 1.0f / (float)N);
     }
-  }
-  void inverse_begin() {
-    for (int ch = 0; ch < nch; ++ch) zero(acc[ch].p);
   }
 };
 
@@ -319,12 +418,15 @@ int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, in
     P->f1 = make_fft_plan(n1, pool);
     P->f2 = make_fft_plan(n2, pool);
     P->f3 = make_fft_plan(n3, pool);
-    P->g_rows = std::max(2, std::min(16, pick_group(n1)));
-    P->g_axis = std::min(pick_group(std::max(n2, n3)), 16);
-    check_lds(P->lds_rows_r2c(), "the axis-1 transform");
-    check_lds(P->lds_rows_c2r(), "the axis-1 transform");
-    check_lds(P->lds_axis(n2), "the axis-2 transform");
-    check_lds(P->lds_axis(n3), "the axis-3 transform");
+    P->fast = std::getenv("SURFHIP_NS_GENERIC") == nullptr && fast_ops_for(n1, n2, n3, P->ops);
+    if (!P->fast) {
+      P->g_rows = std::max(2, std::min(16, pick_group(n1)));
+      P->g_axis = std::min(pick_group(std::max(n2, n3)), 16);
+      check_lds(P->lds_rows_r2c(), "the axis-1 transform");
+      check_lds(P->lds_rows_c2r(), "the axis-1 transform");
+      check_lds(P->lds_axis(n2), "the axis-2 transform");
+      check_lds(P->lds_axis(n3), "the axis-3 transform");
+    }
     P->tw.upload(pool);
     P->hg1.upload(P->T.hg1);
     P->hg1c.upload(P->T.hg1c);
@@ -335,13 +437,21 @@ int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, in
     P->c.upload(P->T.c);
     if (P->T.planes.empty()) P->T.planes.push_back(0.0f);
     P->planes.upload(P->T.planes);
+    P->ditems.upload(P->T.items);
     P->dview = NSMaskView{P->hg1.p, P->hg1c.p, P->hg2.p, P->hg2c.p, P->a.p, P->l.p, P->c.p, P->planes.p,
                           P->h1,    n2,         n3,        P->T.J,    P->T.tight};
     for (int ch = 0; ch < P->nch; ++ch) {
       P->X[ch].alloc(P->H);
       P->acc[ch].alloc(P->H);
     }
-    P->work.alloc(P->H);
+    if (P->fast) {
+      for (int c : P->T.group_count) P->maxw = std::max(P->maxw, c);
+      P->VT.alloc((size_t)P->maxw * P->H);
+      P->Vfin.alloc(P->H);
+      for (int ch = 0; ch < P->nch; ++ch) P->accg[ch].alloc(P->H);
+    } else {
+      P->work.alloc(P->H);
+    }
     // The host copies of the big tables are no longer needed.
     std::vector<float>().swap(P->T.hg1);
     std::vector<float>().swap(P->T.hg1c);
@@ -368,22 +478,14 @@ int surfhip_ns_set_stream(surfhip_ns_plan *plan, void *stream) {
 int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
-    plan->forward_spectrum((const float *)x);
-    float *c = (float *)coeffs;
-    for (int i = 0; i < plan->T.S; ++i) plan->subband(i, c + (int64_t)i * plan->N * plan->nch);
+    plan->analyze((const float *)x, (float *)coeffs);
   });
 }
 
 int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
-    const float *c = (const float *)coeffs;
-    plan->inverse_begin();
-    // Directional subbands in increasing index, then the lowpass: the order of
-    // the streaming synthesis, so both give identical sums.
-    for (int i = 1; i < plan->T.S; ++i) plan->accumulate(i, c + (int64_t)i * plan->N * plan->nch);
-    plan->accumulate(0, c);
-    plan->finish((float *)x);
+    plan->synthesize((const float *)coeffs, (float *)x);
   });
 }
 
@@ -400,7 +502,7 @@ int surfhip_ns_stream_subband(surfhip_ns_plan *plan, int32_t i, void *band) {
     if (!plan || !band) fail(SURFHIP_ERR_ARG, "null argument");
     if (!plan->streaming_fwd) fail(SURFHIP_ERR_ARG, "surfhip_ns_stream_begin was not called");
     if (i < 1 || i > plan->T.S) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", i, plan->T.S));
-    plan->subband(i - 1, (float *)band);
+    plan->subbands(i - 1, 1, (float *)band);
   });
 }
 

@@ -230,7 +230,7 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
       for (int k1 = 0; k1 < h1; ++k1) {
         double f0 = halfband_split(col[k1], poly);
         double f1 = halfband_split(-col[k1], poly);   // = 1 - f0, without cancellation
-        size_t q = (size_t)k1 + (size_t)h1 * ((size_t)k2 + (size_t)n2 * k3);
+        size_t q = ((size_t)k1 * n2 + (size_t)k2) * n3 + k3;
         T.hg1[q] = (float)(T.tight ? std::sqrt(std::max(f0, 0.0)) : f0);
         T.hg1c[q] = (float)(T.tight ? std::sqrt(std::max(f1, 0.0)) : f1);
       }
@@ -246,12 +246,12 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
           if (c != 0.0) m += c * b2[(size_t)j2 * n2 + k2] * b3[(size_t)j3 * n3 + k3];
         }
       double f0 = halfband_split(m, poly), f1 = halfband_split(-m, poly);
-      T.hg2[(size_t)k2 + (size_t)n2 * k3] = (float)(T.tight ? std::sqrt(std::max(f0, 0.0)) : f0);
-      T.hg2c[(size_t)k2 + (size_t)n2 * k3] = (float)(T.tight ? std::sqrt(std::max(f1, 0.0)) : f1);
+      T.hg2[(size_t)k2 * n3 + k3] = (float)(T.tight ? std::sqrt(std::max(f0, 0.0)) : f0);
+      T.hg2c[(size_t)k2 * n3 + k3] = (float)(T.tight ? std::sqrt(std::max(f1, 0.0)) : f1);
     }
 
   // Wedge planes, one pool; each plane is stored over its two axes in
-  // increasing order, (p < q): index kp + len_p * kq.
+  // increasing order (p < q), C-ordered: index kp * len_q + kq.
   T.planes.clear();
   T.items.assign(S, NSItem{});
   T.items[0] = NSItem{1, J, 0, -1, 0, -1, 0, -1};
@@ -287,7 +287,7 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
             for (int k = 0; k < nw; ++k) {
               double w = wedge_weight(sr, k + 1, nw, p.transition, poly);
               if (p.tight) w = std::sqrt(std::max(w, 0.0));
-              T.planes[(size_t)offs[s][k] + ilo + (size_t)lenlo * ihi] = (float)w;
+              T.planes[(size_t)offs[s][k] + (size_t)ilo * lenhi + ihi] = (float)w;
             }
           }
       }

@@ -28,18 +28,20 @@ struct NSItem {
 
 // Raw table pointers (host or device) and the grid.
 struct NSMaskView {
-  const float *hg1, *hg1c;  // (h1, n2, n3): stage-1 factor f0(M1) and f1(M1), sqrt'd when tight
-  const float *hg2, *hg2c;  // (n2, n3): stage-2 factor f0(M2) and f1(M2), sqrt'd when tight
+  // All tables are C-ordered over (k1, k2, k3) -- k3 fastest -- so that a
+  // pencil along axis 3 reads them contiguously.
+  const float *hg1, *hg1c;  // [k1][k2][k3]: stage-1 factor f0(M1) and f1(M1), sqrt'd when tight
+  const float *hg2, *hg2c;  // [k2][k3]: stage-2 factor f0(M2) and f1(M2), sqrt'd when tight
   const float *a;           // (J+1) x (h1+n2+n3): prod_{i<j} l_i per axis
   const float *l;           // J x (h1+n2+n3): l_j per axis
   const float *c;           // J x (h1+n2+n3): 1 - l_j per axis (formed in FP64)
-  const float *planes;      // wedge plane pool (sqrt'd when tight)
+  const float *planes;      // wedge plane pool (sqrt'd when tight), each [lower axis][higher axis]
   int32_t h1, n2, n3, J, tight;
 };
 
 SH_HD inline float ns_plane(const NSMaskView &v, int32_t off, int32_t pair, int k1, int k2, int k3) {
   if (off < 0) return 1.0f;
-  int idx = pair == 0 ? k1 + v.h1 * k2 : (pair == 1 ? k1 + v.h1 * k3 : k2 + v.n2 * k3);
+  int idx = pair == 0 ? k1 * v.n2 + k2 : (pair == 1 ? k1 * v.n3 + k3 : k2 * v.n3 + k3);
   return v.planes[off + idx];
 }
 
@@ -60,10 +62,11 @@ SH_HD inline float ns_bandpass(const NSMaskView &v, int lev, int k1, int k2, int
   return (a[k1] * a[i2] * a[i3]) * sqrtf(r > 0.0f ? r : 0.0f);
 }
 
-// Mask of subband `it` at rfft bin (k1, k2, k3).  analysis = false gives the
-// synthesis mask, which differs only when the bank is not tight (group mask
-// B alone and a unit wedge: src/transforms/ns_surfacelet.jl:226-238).
-SH_HD inline float ns_mask(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
+// Group factor of subband `it` at rfft bin (k1, k2, k3): B_j times the
+// hourglass factor (analysis, and synthesis of a tight bank), B_j alone (the
+// synthesis of a bank that is not tight), or the lowpass a1 a2 a3
+// (src/transforms/ns_surfacelet.jl:226-227,265-272,295-303,350-358).
+SH_HD inline float ns_group(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
   if (it.lowpass) {
     const int L = v.h1 + v.n2 + v.n3;
     const float *a = v.a + (size_t)v.J * L;
@@ -71,12 +74,24 @@ SH_HD inline float ns_mask(const NSMaskView &v, const NSItem &it, int k1, int k2
   }
   float B = ns_bandpass(v, it.level, k1, k2, k3);
   if (!analysis && !v.tight) return B;
-  size_t p3 = (size_t)k1 + (size_t)v.h1 * ((size_t)k2 + (size_t)v.n2 * k3);
-  size_t p2 = (size_t)k2 + (size_t)v.n2 * k3;
+  size_t p3 = ((size_t)k1 * v.n2 + (size_t)k2) * v.n3 + k3;
+  size_t p2 = (size_t)k2 * v.n3 + k3;
   float h = it.hourglass == 1 ? v.hg1[p3] : v.hg1c[p3] * (it.hourglass == 2 ? v.hg2[p2] : v.hg2c[p2]);
-  float wa = ns_plane(v, it.pa_off, it.pa_pair, k1, k2, k3);
-  float wb = ns_plane(v, it.pb_off, it.pb_pair, k1, k2, k3);
-  return (B * h) * (wa * wb);
+  return B * h;
+}
+
+// Wedge factor w_a w_b (src/transforms/ns_surfacelet.jl:376-427); 1 for the
+// lowpass and for the synthesis of a bank that is not tight (unit wedge).
+SH_HD inline float ns_wedge(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
+  if (it.lowpass || (!analysis && !v.tight)) return 1.0f;
+  return ns_plane(v, it.pa_off, it.pa_pair, k1, k2, k3) * ns_plane(v, it.pb_off, it.pb_pair, k1, k2, k3);
+}
+
+// Full mask of subband `it`: group factor times wedge factor.  analysis =
+// false gives the synthesis mask, which differs only when the bank is not
+// tight (src/transforms/ns_surfacelet.jl:226-238).
+SH_HD inline float ns_mask(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
+  return ns_group(v, it, k1, k2, k3, analysis) * ns_wedge(v, it, k1, k2, k3, analysis);
 }
 
 // Host-side tables of one (transform, size).

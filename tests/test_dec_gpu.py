@@ -1,0 +1,158 @@
+"""Decimated Surfacelet on the GPU through libsurfhip.
+
+Checked against SurfBox's golden fingerprints (the reference's own fixtures,
+test/reference/surfbox_irc.*) and against the FP64 oracle.  FP32 tolerance:
+every fingerprint entry within 2e-6 of the array norm (the reference's FP64
+bar is 1e-11); coefficients within relative L2 1e-5 of the oracle.
+Real and complex: complex data is two real channels, bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import decimated as D
+from oracle.golden import fp_error, load_golden
+from oracle.golden import testvol as make_testvol
+
+pytestmark = pytest.mark.gpu
+
+surfhip = pytest.importorskip("surfhip")
+from surfhip import decimated as S  # noqa: E402
+from surfhip._arrays import from_numpy, to_numpy  # noqa: E402
+from surfhip.phantom import ellipsoid_phantom  # noqa: E402
+
+L111 = np.array([[-1, 1, 1], [1, -1, 1], [1, 1, -1]])
+L212 = np.array([[-1, 2, 1], [1, -1, 2], [2, 1, -1]])
+FP_TOL = 2e-6
+RTOL = 1e-5
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return "cuda"
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
+
+
+@pytest.mark.parametrize("tag,n,mode,lev_array,bo", [
+    ("m15", 48, 1.5, [L111, L111], 8),
+    ("m2", 64, 2, [L212, L111], 4),
+])
+def test_surfacelet_matches_surfbox_fingerprints(tag, n, mode, lev_array, bo):
+    # test/items/test_irc.jl:163-184 through the GPU path.
+    dev = _dev()
+    print(f"case {tag} n={n}", flush=True)
+    G = load_golden()
+    x = make_testvol(n).astype(np.float32)
+    t = S.Surfacelet(direction_levels=lev_array[::-1], pyramid_ratio=mode, checkerboard_order=bo)
+    xd = from_numpy(x, dev)
+    c = S.analyze(t, xd)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, d in enumerate(S.subband_layout(t, x.shape), start=1):
+        key = f"surf_{tag}_low" if i == 1 else f"surf_{tag}_{t.scales - d.scale + 1}_{d.hourglass}_{d.wedge}"
+        band = to_numpy(c.subband(i)).astype(np.float64)
+        assert tuple(band.shape) == tuple(G[key]["size"]), key
+        worst = max(worst, fp_error(band, G[key]))
+    assert worst <= FP_TOL, worst
+    r = to_numpy(S.synthesize(t, c)).astype(np.float64)
+    assert fp_error(r, G[f"surfrec_{tag}"]) <= FP_TOL
+    assert np.max(np.abs(r - x)) < 1e-5
+
+
+CASES = [
+    ((48, 48, 48), 1.5, 2, [1, 2], 12),
+    ((48, 24, 36), 1.5, 1, [2], 8),
+    ((32, 32, 32), 1, 2, [1, 1], 4),
+    ((64, 32, 32), 2, 2, [1, 3], 18),
+    ((32, 32, 32), 2, 1, [np.array([[-1, 2, 0], [1, -1, 1], [0, 3, -1]])], 6),
+    ((24, 24, 24), 1.5, 1, [0], 12),
+]
+
+
+@pytest.mark.parametrize("dims,mode,J,L,bo", CASES)
+def test_surfacelet_matches_oracle(dims, mode, J, L, bo):
+    # test/items/test_surfacelet.jl:43-63's matrix of modes, orders and levels.
+    dev = _dev()
+    print(f"case dims={dims} mode={mode} J={J} bo={bo}", flush=True)
+    rng = np.random.default_rng(sum(dims) * 7 + J)
+    x = rng.standard_normal(dims).astype(np.float32)
+    t = S.Surfacelet(scales=J, direction_levels=L, pyramid_ratio=mode, checkerboard_order=bo)
+    to = D.SParams(scales=J, direction_levels=tuple(L), pyramid_ratio=mode, checkerboard_order=bo)
+    ws_o = D.workspace(to, dims)
+    ref = D.analyze(to, x.astype(np.float64), ws_o)
+    xd = from_numpy(x, dev)
+    ws = S.make_workspace(t, xd)
+    c = S.analyze(t, xd, workspace=ws)
+    torch.cuda.synchronize()
+    got = to_numpy(c.data).astype(np.float64)
+    flat_ref = np.concatenate([b.ravel(order="F") for b in ref])
+    assert _rel(got, flat_ref) <= RTOL
+    assert len(c) == round(S.surfacelet_redundancy(t, dims) * np.prod(dims))
+    # synthesis of random coefficients against the oracle's synthesis
+    r = rng.standard_normal(c.data.numel()).astype(np.float32)
+    rv = S.SubbandVector(torch.from_numpy(r).to(dev), ws.layout)
+    y = to_numpy(S.synthesize(t, rv, workspace=ws))
+    rb = [r[o:o + int(np.prod(s))].reshape(s, order="F").astype(np.float64)
+          for o, s in zip(ws.layout.offsets, ws.layout.sizes)]
+    yref = D.synthesize(to, rb, ws_o)
+    assert _rel(y, yref) <= RTOL
+    xr = to_numpy(S.synthesize(t, c, workspace=ws))
+    assert _rel(xr, x) <= RTOL
+
+
+def test_surfacelet_complex_is_two_real_channels_bitwise():
+    # test/items/test_surfacelet.jl:91-111
+    dev = _dev()
+    rng = np.random.default_rng(102)
+    xr = rng.standard_normal((48, 24, 36)).astype(np.float32)
+    xi = rng.standard_normal((48, 24, 36)).astype(np.float32)
+    t = S.Surfacelet(scales=2, direction_levels=[1, 2])
+    cc = S.analyze(t, from_numpy((xr + 1j * xi).astype(np.complex64), dev))
+    cr = S.analyze(t, from_numpy(xr, dev))
+    ci = S.analyze(t, from_numpy(xi, dev))
+    torch.cuda.synchronize()
+    assert torch.equal(cc.data, torch.complex(cr.data, ci.data))
+    z = S.synthesize(t, cc)
+    assert _rel(to_numpy(z), xr + 1j * xi) <= RTOL
+    assert torch.equal(z, torch.complex(S.synthesize(t, cr), S.synthesize(t, ci)))
+
+
+def test_surfacelet_validation_on_device():
+    dev = _dev()
+    t = S.Surfacelet(scales=1, direction_levels=[1])
+    ws = S.make_workspace(t, torch.float32, (24, 24, 24))
+    with pytest.raises(surfhip.DimensionMismatch):
+        S.analyze(t, torch.zeros(48, 48, 48, device=dev), workspace=ws)
+    with pytest.raises(surfhip.ArgumentError, match=r"the workspace was built for Surfacelet\(scales = 1"):
+        S.analyze(S.Surfacelet(scales=1, direction_levels=[2]), torch.zeros(24, 24, 24, device=dev), workspace=ws)
+    c = S.analyze(t, from_numpy(np.random.default_rng(104).standard_normal((24, 24, 24)).astype(np.float32), dev),
+                  workspace=ws)
+    other = S.similar_coefficients(S.Surfacelet(scales=1, direction_levels=[2]), torch.zeros(24, 24, 24, device=dev))
+    with pytest.raises(surfhip.DimensionMismatch, match="subband layout"):
+        S.synthesize(t, other, workspace=ws)
+
+
+def test_surfacelet_128_phantom_config():
+    # The decimated workload at its full size: Surfacelet(scales = 3,
+    # direction_levels = [2, 2, 3], pyramid_ratio = 2) on a 128^3 ellipsoid
+    # phantom; the default ratio 1.5 does not fit 128 (nearest: 132).
+    dev = _dev()
+    dims = (128, 128, 128)
+    print("case decimated 128^3 phantom", flush=True)
+    x = ellipsoid_phantom(dims, seed=2026)
+    t = S.Surfacelet(scales=3, direction_levels=[2, 2, 3], pyramid_ratio=2)
+    to = D.SParams(scales=3, direction_levels=(2, 2, 3), pyramid_ratio=2)
+    xd = from_numpy(x, dev)
+    ws = S.make_workspace(t, xd)
+    c = S.analyze(t, xd, workspace=ws)
+    torch.cuda.synchronize()
+    assert len(c) == 7180288
+    ref = D.analyze(to, x.astype(np.float64))
+    flat_ref = np.concatenate([b.ravel(order="F") for b in ref])
+    assert _rel(to_numpy(c.data), flat_ref) <= RTOL
+    xr = to_numpy(S.synthesize(t, c, workspace=ws))
+    assert _rel(xr, x) <= RTOL

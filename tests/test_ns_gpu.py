@@ -52,6 +52,12 @@ CASES = [
     ((16, 12, 20), 2, (1, 2), 6, False),
     ((26, 26, 26), 2, (2, 2), 12, True),
     ((32, 24, 40), 3, (2, 2, 3), 12, True),
+    # power-of-two grids take the fused fast path
+    ((32, 32, 32), 2, (1, 2), 6, True),
+    ((64, 32, 128), 3, (1, 2, 3), 12, True),
+    ((128, 64, 32), 2, (2, 3), 12, True),
+    ((32, 128, 64), 3, (2, 2, 3), 12, False),
+    ((64, 64, 256), 2, (1, 2), 6, True),
 ]
 
 
@@ -79,12 +85,17 @@ def test_ns_analyze_synthesize_match_oracle(dims, J, L, bo, tight):
     # perfect reconstruction
     xr = N.synthesize(t, c, workspace=ws)
     assert _rel(to_numpy(xr), x) <= RTOL
+    # streaming at full size equals the materialised transform bit for bit
+    y2 = torch.empty_like(xr)
+    N.synthesize_(lambda i: N.subband(t, c, i), y2, t, workspace=ws)
+    assert torch.equal(xr, y2)
 
 
-def test_ns_streaming_matches_materialised_bitwise():
+@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 64, 32)])
+def test_ns_streaming_matches_materialised_bitwise(dims):
     dev = _dev()
     t = N.NSSurfacelet(scales=2, direction_levels=[1, 2], bernstein_order=6)
-    x = from_numpy(np.random.default_rng(92).standard_normal((16, 16, 16)).astype(np.float32), dev)
+    x = from_numpy(np.random.default_rng(92).standard_normal(dims).astype(np.float32), dev)
     ws = N.make_workspace(t, x)
     ref = N.analyze(t, x, workspace=ws)
     seen = []
@@ -94,7 +105,7 @@ def test_ns_streaming_matches_materialised_bitwise():
         assert torch.equal(band, N.subband(t, ref, i))
 
     assert N.analyze(f, t, x, workspace=ws) is None
-    assert seen == list(range(2, N.nsubbands(t, (16, 16, 16)) + 1)) + [1]
+    assert seen == list(range(2, N.nsubbands(t, dims) + 1)) + [1]
     y1 = N.synthesize(t, ref, workspace=ws)
     y2 = torch.empty_like(y1)
     N.synthesize_(lambda i: N.subband(t, ref, i), y2, t, workspace=ws)
@@ -103,11 +114,12 @@ def test_ns_streaming_matches_materialised_bitwise():
         N.synthesize_(lambda i: torch.zeros(4, 4, 4, device=dev), torch.empty_like(y1), t, workspace=ws)
 
 
-def test_ns_complex_is_two_real_channels_bitwise():
+@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 32, 64)])
+def test_ns_complex_is_two_real_channels_bitwise(dims):
     dev = _dev()
     rng = np.random.default_rng(94)
-    xr = rng.standard_normal((16, 16, 16)).astype(np.float32)
-    xi = rng.standard_normal((16, 16, 16)).astype(np.float32)
+    xr = rng.standard_normal(dims).astype(np.float32)
+    xi = rng.standard_normal(dims).astype(np.float32)
     t = N.NSSurfacelet(scales=2, direction_levels=[1, 1], bernstein_order=6)
     z = from_numpy((xr + 1j * xi).astype(np.complex64), dev)
     cc = N.analyze(t, z)
@@ -122,11 +134,12 @@ def test_ns_complex_is_two_real_channels_bitwise():
     assert torch.equal(zr, torch.complex(yr, yi))
 
 
-def test_ns_workspace_reuse_bitwise_and_validated():
+@pytest.mark.parametrize("n", [16, 32])
+def test_ns_workspace_reuse_bitwise_and_validated(n):
     dev = _dev()
     t = N.NSSurfacelet(scales=2, direction_levels=[1, 2], bernstein_order=6)
-    x = from_numpy(np.random.default_rng(96).standard_normal((16, 16, 16)).astype(np.float32), dev)
-    ws = N.make_workspace(t, torch.float32, (16, 16, 16))
+    x = from_numpy(np.random.default_rng(96).standard_normal((n, n, n)).astype(np.float32), dev)
+    ws = N.make_workspace(t, torch.float32, (n, n, n))
     a = N.analyze(t, x)
     b = N.analyze(t, x, workspace=ws)
     c = N.analyze(ws, x)
@@ -135,26 +148,28 @@ def test_ns_workspace_reuse_bitwise_and_validated():
         N.analyze(t, torch.zeros(8, 8, 8, device=dev), workspace=ws)
     with pytest.raises(surfhip.ArgumentError, match="element type"):
         N.analyze(t, torch.complex(x, x), workspace=ws)
-    wsj = N.make_workspace(N.NSSurfacelet(scales=1, direction_levels=[1]), torch.float32, (16, 16, 16))
+    wsj = N.make_workspace(N.NSSurfacelet(scales=1, direction_levels=[1]), torch.float32, (n, n, n))
     with pytest.raises(surfhip.ArgumentError, match=r"the workspace was built for NSSurfacelet\(scales = 1"):
         N.analyze(t, x, workspace=wsj)
     with pytest.raises(surfhip.ArgumentError, match="tasks must be ≥ 1"):
-        N.make_workspace(t, torch.float32, (16, 16, 16), tasks=0)
+        N.make_workspace(t, torch.float32, (n, n, n), tasks=0)
 
 
-def test_ns_constant_volume_stays_in_lowpass():
+@pytest.mark.parametrize("n", [16, 32])
+def test_ns_constant_volume_stays_in_lowpass(n):
     dev = _dev()
     t = N.NSSurfacelet(scales=2, direction_levels=[1, 1], bernstein_order=6)
-    c = N.analyze(t, from_numpy(np.full((16, 16, 16), 1.75, np.float32), dev))
+    c = N.analyze(t, from_numpy(np.full((n, n, n), 1.75, np.float32), dev))
     got = to_numpy(c)
     assert np.max(np.abs(got[..., 0] - 1.75)) < 1e-5
     assert np.max(np.abs(got[..., 1:])) < 1e-5
 
 
-def test_ns_shift_equivariance():
+@pytest.mark.parametrize("n", [16, 32])
+def test_ns_shift_equivariance(n):
     dev = _dev()
     t = N.NSSurfacelet(scales=2, direction_levels=[1, 2], bernstein_order=6)
-    x = np.random.default_rng(91).standard_normal((16, 16, 16)).astype(np.float32)
+    x = np.random.default_rng(91).standard_normal((n, n, n)).astype(np.float32)
     s = (3, -2, 5)
     a = to_numpy(N.analyze(t, from_numpy(x, dev)))
     b = to_numpy(N.analyze(t, from_numpy(np.roll(x, s, axis=(0, 1, 2)), dev)))
@@ -183,3 +198,7 @@ def test_ns_128_default_transform_matches_oracle():
     assert abs(np.sum(cn ** 2) / np.sum(x.astype(np.float64) ** 2) - 1) < 1e-5
     xr = N.synthesize(t, c, workspace=ws)
     assert _rel(to_numpy(xr), x) <= RTOL
+    # streaming at full size equals the materialised transform bit for bit
+    y2 = torch.empty_like(xr)
+    N.synthesize_(lambda i: N.subband(t, c, i), y2, t, workspace=ws)
+    assert torch.equal(xr, y2)
