@@ -7,9 +7,14 @@ bernstein_order = 12, halfband_order = 1, transition = 1, tight) on a
 workspace; the input and the coefficients stay resident in HBM.
 
 metric: voxel-subbands per second = n1 n2 n3 S / (seconds per step).
-roofline: algorithmic HBM bytes of one step, 8 N (S + 1) (read x, write S
-subbands; read S subbands, write x), over the step's duration measured with
-HIP events on the stream every kernel is launched on, against 8 TB/s.
+roofline: the dominant kernel class, timed live with HIP events that the
+library records around each of its launches on the stream they run on
+(a second, instrumented loop; the headline ms_per_step comes from an
+uninstrumented loop).  The classes that touch the user's data carry the
+algorithmic bytes: the analysis plane pass writes the S subbands (4 N bytes
+each) and the synthesis plane pass reads them; achieved = those bytes over
+the class's summed launch time, against 8 TB/s.  The whole step's
+algorithmic bytes, 8 N (S + 1), over ms_per_step are reported beside it.
 cpu_baseline: the FP32 CPU restatement of the reference's schedule
 (oracle/cpu_schedule.py) on the host cores, on a bounded sample.
 """
@@ -24,6 +29,10 @@ sys.path.insert(0, os.path.join(ROOT, "surfacelets.jl_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the plane kernels, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs (profiles/), scaled by a copy-kernel calibration;
+# None until measured.
+TRAFFIC = {}
 DIMS = (128, 128, 128)
 SEED = 20260
 
@@ -36,7 +45,22 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU work budget of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """One plain device-to-device copy of a known size, for scaling the HBM
+    byte counters of rocprofv3 --pmc runs."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "read_bytes": n * 4, "write_bytes": n * 4}), flush=True)
 
 
 def cpu_baseline(x_np, budget_s):
@@ -67,6 +91,9 @@ def cpu_baseline(x_np, budget_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the surfacelet path here runs on a single MI355X "
               "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
@@ -107,6 +134,20 @@ def main():
     ms = ev0.elapsed_time(ev1) / args.steps
     rel = float(torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x))
 
+    # Instrumented loop: per-class kernel times from events around each launch.
+    ws.set_timing(True)
+    for _ in range(args.steps):
+        step()
+    cls = ws.timing()
+    ws.set_timing(False)
+    kernels = {k: {"launches_per_step": n / args.steps, "ms_per_step": t / args.steps} for k, (n, t) in cls.items()}
+    # Algorithmic HBM bytes per step of each class (the user's data only).
+    alg = {"plane_analysis": 4 * N * S, "plane_synthesis": 4 * N * S}
+    dom = max(cls, key=lambda k: cls[k][1])
+    dom_plane = max(alg, key=lambda k: cls.get(k, (0, 0.0))[1])
+    launches, tms = cls[dom_plane]
+    dom_achieved = alg[dom_plane] * args.steps / (tms * 1e-3) / 1e9 if tms > 0 else None
+
     bytes_per_step = 8 * N * (S + 1)
     achieved = bytes_per_step / (ms * 1e-3) / 1e9
     result = {
@@ -125,10 +166,14 @@ def main():
         "config": {"workload": "NSSurfacelet(scales=3, direction_levels=[2,2,3]) analyze!+synthesize!, "
                                "128x128x128 Float32, 49 subbands, prebuilt workspace",
                    "dims": list(DIMS), "subbands": S, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "scope": "one analyze!+synthesize! step (all its kernels, one stream), "
-                              "algorithmic bytes 8 N (S+1)"},
+        "roofline": {"bound": "hbm", "achieved": dom_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (dom_achieved / HBM_PEAK_GBS) if dom_achieved else None, "traffic": TRAFFIC.get(dom_plane),
+                     "kernel": dom_plane, "launches_per_step": launches / args.steps,
+                     "avg_launch_us": tms / launches * 1e3 if launches else None,
+                     "dominant_class": dom},
+        "step_roofline": {"achieved": achieved, "frac": achieved / HBM_PEAK_GBS,
+                          "bytes_per_step": bytes_per_step},
+        "kernels": kernels,
         "check": {"reconstruction_rel_l2": rel},
     }
     if not args.no_cpu:

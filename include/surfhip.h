@@ -104,6 +104,17 @@ int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs);
 /* synthesize!: src/transforms/ns_surfacelet.jl:568-581. */
 int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x);
 
+/* Adjoints for reverse-mode rules (no counterpart functions in the
+ * reference; the north-star rrule).  Each subband operator
+ * irfft . M_k . rfft is self-adjoint (M_k real and even), so
+ *   surfhip_ns_adjoint:            x = A^T c, the accumulation with the analysis
+ *                                  masks (= synthesize! for a tight bank:
+ *                                  src/transforms/ns_surfacelet.jl:21-25);
+ *   surfhip_ns_synthesize_adjoint: c = S^T x, the analysis with the synthesis
+ *                                  masks (= analyze! for a tight bank). */
+int surfhip_ns_adjoint(surfhip_ns_plan *plan, const void *coeffs, void *x);
+int surfhip_ns_synthesize_adjoint(surfhip_ns_plan *plan, const void *x, void *coeffs);
+
 /* Streaming analysis, analyze(f, t, x): src/transforms/ns_surfacelet.jl:
  * 474-479,620-624.  stream_begin computes the spectrum of x; each
  * stream_subband(i) then writes subband i (1-based, subband_layout order)
@@ -124,6 +135,18 @@ int surfhip_ns_stream_finish(surfhip_ns_plan *plan, void *x);
  * ndfb_mask (src/directional/hourglass.jl:264-270). */
 int surfhip_ns_mask_host(const surfhip_ns_params *p, int32_t n1, int32_t n2,
                          int32_t n3, int32_t i, float *out);
+
+/* Per-kernel timing (no counterpart in the reference: a profiling hook for
+ * benchmarks).  With timing on, every kernel launch of the plan is bracketed
+ * by HIP events on the plan's stream.  surfhip_ns_timing synchronises the
+ * stream and returns, per kernel class, the number of launches and their
+ * summed duration in milliseconds since timing was last switched on; class
+ * names are written comma-separated into `names`.  Classes: 0 pencil
+ * analysis, 1 plane analysis (subband write), 2 plane synthesis (subband
+ * read), 3 pencil synthesis, 4 other. */
+#define SURFHIP_NS_TIMING_CLASSES 5
+int surfhip_ns_set_timing(surfhip_ns_plan *plan, int32_t enable);
+int surfhip_ns_timing(surfhip_ns_plan *plan, int64_t *launches, double *ms, char *names, size_t n);
 
 /* ------------------------------------------------------ Surfacelet (dec.) */
 
@@ -150,6 +173,12 @@ int surfhip_s_set_stream(surfhip_s_plan *plan, void *stream);
 /* analyze! / synthesize!: src/transforms/surfacelet.jl:446-464. */
 int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs);
 int surfhip_s_synthesize(surfhip_s_plan *plan, const void *coeffs, void *x);
+
+/* x = A^T c, the adjoint of analyze!.  The decimated transform is a frame but
+ * not a tight one (src/transforms/surfacelet.jl:24-26), so this differs from
+ * synthesize!; it is the backward of analyze in a reverse-mode rule (the
+ * reference's north-star rrule).  No counterpart function in the reference. */
+int surfhip_s_adjoint(surfhip_s_plan *plan, const void *coeffs, void *x);
 
 #ifdef __cplusplus
 }

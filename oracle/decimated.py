@@ -497,3 +497,80 @@ def synthesize(t, c, ws=None, dims=None):
         X = scale_inverse(bands, sc)
         C = pyramid_merge(X, C, ws.pyramid[j])
     return np.real(np.fft.ifftn(C))
+
+
+# ------------------------------------------------------------ adjoint -------
+#
+# The decimated transform is a frame but not a tight one, so its adjoint is
+# not `synthesize` (src/transforms/surfacelet.jl:24-26).  The adjoint below is
+# derived stage by stage from the analysis (real inner products; a complex
+# spectrum counts as R^2): fft -> bfft real part, pyramid crop/scale ->
+# scatter with L / D^1.5, IRC 2x2 mixing -> conjugate transpose in reverse
+# stage order, real masks -> the same masks, and the unnormalised c2r of a
+# wedge block -> rfft with the interior bins along K doubled.  It is pinned
+# by dense probes <A x, y> = <x, A^T y> (tests/test_oracle_decimated.py).
+
+def pyramid_split_adjoint(Xh, low, lv):
+    """Adjoint of pyramid_split: returns X-bar from (highpass-bar, lowpass-bar)."""
+    L = _sep(lv.prof)
+    D15 = float(lv.D) ** 1.5
+    ix = np.ix_(*lv.idx)
+    X = Xh.copy()
+    X[ix] = np.sqrt(1 - L ** 2) * Xh[ix] + (L / D15) * low[np.ix_(*lv.idx_low)]
+    return X
+
+
+def irc_analysis_adjoint(S, stages):
+    """Adjoint of irc_analysis: conjugate-transposed stage matrices, reversed."""
+    for st in reversed(stages):
+        iA, iB = _halves_index(S.shape[st.m], st.b)
+        F0a, F0b = _take(st.ana[0], st.m, iA), _take(st.ana[0], st.m, iB)
+        F1a, F1b = _take(st.ana[1], st.m, iA), _take(st.ana[1], st.m, iB)
+        irc_stage(S, st.m, st.b, np.conj(F0a), np.conj(F1a), np.conj(F0b), np.conj(F1b))
+    return S
+
+
+def brfft_K_adjoint(sb, K):
+    """Adjoint of brfft_K (unnormalised c2r along K after inverse c2c along the
+    other axes): rfft over (others, K), interior bins along K doubled."""
+    Z = rfft_K(sb, K)
+    h = Z.shape[K]
+    n = sb.shape[K]
+    sl = [slice(None)] * 3
+    sl[K] = slice(1, h - 1 if n % 2 == 0 else h)
+    Z[tuple(sl)] *= 2
+    return Z
+
+
+def scale_adjoint(subbands, sc):
+    """Adjoint of scale_forward: the highpass spectrum-bar of one scale."""
+    X = np.zeros(sc.dims, dtype=np.complex128)
+    n = 0
+    for K in range(3):
+        counts, bsize = surf_blocks(sc, K)
+        S = np.zeros(half_dims(sc.dims, K), dtype=np.complex128)
+        for i in range(int(np.prod(counts))):
+            S[block_ranges(counts, bsize, i)] = brfft_K_adjoint(subbands[n], K)
+            n += 1
+        irc_analysis_adjoint(S, sc.stages[K])
+        s = 1.0 / np.prod(surf_subband_size(sc, K))
+        hd = half_dims(sc.dims, K)
+        X[:hd[0], :hd[1], :hd[2]] += S * sc.masks[K] * s
+    return X
+
+
+def adjoint(t, c, ws=None, dims=None):
+    """A^T c for the analysis operator A of the decimated Surfacelet."""
+    if any(np.iscomplexobj(b) for b in c):
+        a = adjoint(t, [np.real(b) for b in c], ws, dims)
+        b = adjoint(t, [np.imag(b) for b in c], ws, dims)
+        return a + 1j * b
+    if ws is None:
+        ws = workspace(t, dims)
+    C = np.fft.fftn(np.asarray(c[0], dtype=np.complex128)) / np.prod(c[0].shape)
+    for j in reversed(range(len(ws.scales))):
+        sc = ws.scales[j]
+        nb = n_scale_subbands(sc.levels)
+        Xh = scale_adjoint(c[ws.offsets[j]:ws.offsets[j] + nb], sc)
+        C = pyramid_split_adjoint(Xh, C, ws.pyramid[j])
+    return np.real(np.fft.ifftn(C)) * np.prod(C.shape)

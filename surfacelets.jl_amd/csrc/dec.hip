@@ -104,10 +104,10 @@ __global__ void k_dzero(float2 *__restrict__ p, int64_t n) {
     p[i] = make_float2(0.0f, 0.0f);
 }
 
-// X = complex(x), x read with element stride `is`.
-__global__ void k_to_complex(const float *__restrict__ x, int64_t is, float2 *__restrict__ X, int64_t n) {
+// X = complex(x) * scale, x read with element stride `is`.
+__global__ void k_to_complex(const float *__restrict__ x, int64_t is, float2 *__restrict__ X, int64_t n, float scale) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    X[i] = make_float2(x[i * is], 0.0f);
+    X[i] = make_float2(x[i * is] * scale, 0.0f);
 }
 
 // out[i * os] = Re(X[i]) * scale.
@@ -154,8 +154,9 @@ __global__ void k_pyr_split(float2 *__restrict__ X, float2 *__restrict__ low, Py
   }
 }
 
-// X[idx] = sqrt(1 - L^2) X[idx] + L D^1.5 low[idx_low].
-__global__ void k_pyr_merge(float2 *__restrict__ X, const float2 *__restrict__ low, PyrDev P) {
+// X[idx] = sqrt(1 - L^2) X[idx] + L fac low[idx_low]: fac = D^1.5 merges
+// (synthesis); fac = 1 / D^1.5 is the adjoint of the split.
+__global__ void k_pyr_merge(float2 *__restrict__ X, const float2 *__restrict__ low, PyrDev P, float fac) {
   const int64_t n = (int64_t)P.len1 * P.len2 * P.len3;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     const int a = e % P.len1;
@@ -165,7 +166,7 @@ __global__ void k_pyr_merge(float2 *__restrict__ X, const float2 *__restrict__ l
     float L, s;
     pyr_factors(P, a, b, c, xi, li, L, s);
     const float2 v = X[xi], w = low[li];
-    const float f = L * P.D15;
+    const float f = L * fac;
     X[xi] = make_float2(s * v.x + f * w.x, s * v.y + f * w.y);
   }
 }
@@ -185,10 +186,12 @@ __global__ void k_masked(const float2 *__restrict__ X, int n1, int n2, float2 *_
 }
 
 // One IRC stage on S (dims h, hourglass axis K, decimated axis m, block b):
-// analysis A <- F0(A) A + F0(B) B, B <- F1(A) A + F1(B) B; synthesis
-// A <- G0(A) A + G1(A) B, B <- G0(B) A + G1(B) B (irc.jl:105-146).
+// mode 0 analysis A <- F0(A) A + F0(B) B, B <- F1(A) A + F1(B) B;
+// mode 1 synthesis A <- G0(A) A + G1(A) B, B <- G0(B) A + G1(B) B
+// (irc.jl:105-146); mode 2 the adjoint of analysis, the conjugate transpose
+// A <- conj F0(A) A + conj F1(A) B, B <- conj F0(B) A + conj F1(B) B.
 __global__ void k_irc(float2 *__restrict__ S, int h1, int h2, int h3, int K, int m, int b,
-                      const float2 *__restrict__ f0, const float2 *__restrict__ f1, int synthesis) {
+                      const float2 *__restrict__ f0, const float2 *__restrict__ f1, int mode) {
   const int hd[3] = {h1, h2, h3};
   const int nm = hd[m], nKh = hd[K];
   const int64_t npairs = (int64_t)h1 * h2 * h3 / 2;
@@ -212,12 +215,15 @@ __global__ void k_irc(float2 *__restrict__ S, int h1, int h2, int h3, int K, int
     const float2 f0A = f0[kK + (int64_t)nKh * cA], f0B = f0[kK + (int64_t)nKh * cB];
     const float2 f1A = f1[kK + (int64_t)nKh * cA], f1B = f1[kK + (int64_t)nKh * cB];
     float2 nA, nB;
-    if (!synthesis) {
+    if (mode == 0) {
       nA = cadd(cmul(f0A, A), cmul(f0B, B));
       nB = cadd(cmul(f1A, A), cmul(f1B, B));
-    } else {
+    } else if (mode == 1) {
       nA = cadd(cmul(f0A, A), cmul(f1A, B));
       nB = cadd(cmul(f0B, A), cmul(f1B, B));
+    } else {
+      nA = cadd(cmul(conjf2(f0A), A), cmul(conjf2(f1A), B));
+      nB = cadd(cmul(conjf2(f0B), A), cmul(conjf2(f1B), B));
     }
     S[eA] = nA;
     S[eB] = nB;
@@ -340,8 +346,11 @@ __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, 
 }
 
 // S lines = r2c along K of the subbands' lines (the first nK/2+1 bins).
+// With `dbl` the interior bins 1 .. nK/2 - 1 are doubled: the adjoint of the
+// unnormalised c2r along K.
 __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, int64_t is, float2 *__restrict__ S,
-                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg,
+                                                 int dbl) {
   extern __shared__ float2 smem[];
   const int nK = M.nK, hK = M.h[M.K];
   float2 *tw = smem;
@@ -383,7 +392,9 @@ __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, 
     if (g >= valid) continue;
     int o[3];
     line_index(M, l0 + g, o);
-    S[half_elem(M, o, t)] = res[t * G + g];
+    float2 v = res[t * G + g];
+    if (dbl && t >= 1 && t < nK - t) v = cscale(v, 2.0f);
+    S[half_elem(M, o, t)] = v;
   }
 }
 
@@ -500,7 +511,7 @@ struct surfhip_s_plan {
     M.off = sc[j]->boff[K].p;
     return M;
   }
-  void lines(bool c2r, int j, int K, float2 *S, float *coef, int64_t stride) {
+  void lines(bool c2r, int j, int K, float2 *S, float *coef, int64_t stride, int dbl = 0) {
     BlockMap M = block_map(j, K);
     const int nK = M.nK, hK = M.h[K];
     const FftPlan1D &pl = plan_for(nK);
@@ -513,7 +524,7 @@ struct surfhip_s_plan {
       hipLaunchKernelGGL(k_c2r_lines, dim3(blocks), dim3(DT), lds, stream, S, coef, stride, M, G, pl, tw.p);
     else
       hipLaunchKernelGGL(k_r2c_lines, dim3(blocks), dim3(DT), lds, stream, (const float *)coef, stride, S, M, G, pl,
-                         tw.p);
+                         tw.p, dbl);
     SH_LAUNCHED();
   }
 
@@ -522,7 +533,7 @@ struct surfhip_s_plan {
     Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
     float2 *X = spectra[0].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, stream, x, (int64_t)nch, X, N);
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, stream, x, (int64_t)nch, X, N, 1.0f);
     SH_LAUNCHED();
     fft3(-1, X, n);
     for (int j = 0; j < T.J; ++j) {
@@ -569,7 +580,8 @@ struct surfhip_s_plan {
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, stream, coef + T.offsets[0] * nch, is, C, NL);
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, stream, coef + T.offsets[0] * nch, is, C, NL,
+                       1.0f);
     SH_LAUNCHED();
     fft3(-1, C, nl);
     for (int j = T.J - 1; j >= 0; --j) {
@@ -604,7 +616,7 @@ struct surfhip_s_plan {
       }
       const PyrDev &pd = lev[j]->pd;
       hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, stream, X,
-                         spectra[j + 1].p, pd);
+                         spectra[j + 1].p, pd, pd.D15);
       SH_LAUNCHED();
     }
     const Dims3 n = T.dims;
@@ -612,6 +624,59 @@ struct surfhip_s_plan {
     fft3(1, spectra[0].p, n);
     hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, stream, spectra[0].p, x, (int64_t)nch,
                        (float)(1.0 / (double)N), N);
+    SH_LAUNCHED();
+  }
+
+  // x = A^T c, the adjoint of the analysis (not its inverse: the transform
+  // is a frame but not a tight one, src/transforms/surfacelet.jl:24-26).
+  void adjoint_channel(const float *coef, float *x) {
+    const int64_t is = nch;
+    const Dims3 nl = T.pyr.back().nlow;
+    const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
+    float2 *C = spectra[T.J].p;
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, stream, coef + T.offsets[0] * nch, is, C, NL,
+                       (float)(1.0 / (double)NL));
+    SH_LAUNCHED();
+    fft3(-1, C, nl);
+    for (int j = T.J - 1; j >= 0; --j) {
+      const DecScaleT &s = T.scales[j];
+      float2 *X = spectra[j].p;
+      const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
+      for (int K = 0; K < 3; ++K) {
+        Dims3 hd = half_dims(s.n, K);
+        float2 *S = Sbuf[K].p;
+        lines(false, j, K, S, const_cast<float *>(coef), is, 1);
+        Dims3 counts, bsize;
+        scale_blocks(s, K, counts, bsize);
+        for (int d = 0; d < 3; ++d)
+          if (d != K) launch_c2c(d, -1, S, hd, bsize[d]);
+        const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
+        for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
+          const IRCStageT &st = s.stages[K][q];
+          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, stream, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+                             filt.p + st.off[0], filt.p + st.off[1], 2);
+          SH_LAUNCHED();
+        }
+      }
+      zero(X, NX);
+      for (int K = 0; K < 3; ++K) {
+        Dims3 hd = half_dims(s.n, K);
+        Dims3 sb = subband_size(s, K);
+        const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
+        const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
+        hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, stream, X, s.n[0], s.n[1], s.n[2],
+                           Sbuf[K].p, masks.p + s.mask_off[K], K, hd[0], hd[1], hd[2], scale, 0);
+        SH_LAUNCHED();
+      }
+      const PyrDev &pd = lev[j]->pd;
+      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, stream, X,
+                         spectra[j + 1].p, pd, 1.0f / pd.D15);
+      SH_LAUNCHED();
+    }
+    const Dims3 n = T.dims;
+    const int64_t N = (int64_t)n[0] * n[1] * n[2];
+    fft3(1, spectra[0].p, n);
+    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, stream, spectra[0].p, x, (int64_t)nch, 1.0f, N);
     SH_LAUNCHED();
   }
 };
@@ -775,6 +840,13 @@ int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
     for (int ch = 0; ch < plan->nch; ++ch) plan->analyze_channel((const float *)x + ch, (float *)coeffs + ch);
+  });
+}
+
+int surfhip_s_adjoint(surfhip_s_plan *plan, const void *coeffs, void *x) {
+  return guarded([&] {
+    if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    for (int ch = 0; ch < plan->nch; ++ch) plan->adjoint_channel((const float *)coeffs + ch, (float *)x + ch);
   });
 }
 

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -206,14 +207,74 @@ struct surfhip_ns_plan {
   FastOps ops;
   int maxw = 1;                  // largest group
   DevBuf<float2> VT, Vfin, accg[2];
+  // optional per-class event timing
+  bool timing = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> tev;
+  int64_t tcount[SURFHIP_NS_TIMING_CLASSES] = {0};
+  double tms[SURFHIP_NS_TIMING_CLASSES] = {0};
   int open_group = -2;           // streaming synthesis: group being accumulated (-2: none)
   int open_item = 0;             // an item of that group
   bool open_fresh = true;        // no term accumulated yet in the open group
+
+  // Run `launch` bracketed by events of class `cls` when timing is on.
+  template <class F> void timed(int cls, F &&launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    hipEvent_t a, b;
+    SH_HIP(hipEventCreate(&a));
+    SH_HIP(hipEventCreate(&b));
+    SH_HIP(hipEventRecord(a, stream));
+    launch();
+    SH_HIP(hipEventRecord(b, stream));
+    tev.push_back({cls, {a, b}});
+  }
+  void collect_timing() {
+    if (tev.empty()) return;
+    SH_HIP(hipStreamSynchronize(stream));
+    for (auto &e : tev) {
+      float ms = 0.0f;
+      SH_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+      tcount[e.first] += 1;
+      tms[e.first] += ms;
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+    tev.clear();
+  }
+  ~surfhip_ns_plan() {
+    for (auto &e : tev) {
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+  }
 
   FastCtx ctx() const {
     return FastCtx{stream, n1, n2, n3, h1, H, dview, ditems.p, tw.p + f1.tw, tw.p + f2.tw, tw.p + f3.tw};
   }
   int group_of(int i) const { return T.items[i].lowpass ? -1 : T.items[i].group; }
+  int group_size(int i) const { return T.items[i].lowpass ? 1 : T.group_count[T.items[i].group]; }
+  // Frequency support of each group (and the lowpass): see Support.
+  std::vector<Support> gsup;
+  Support lowsup{}, fullsup{};
+  Support support_of(int i) const { return T.items[i].lowpass ? lowsup : gsup[T.items[i].group]; }
+  Support level_support(int lev) const {
+    const int L = h1 + n2 + n3;
+    const float *a = T.a.data() + (size_t)lev * L;
+    int K1 = 0, K2 = 0;
+    for (int k = 0; k < h1; ++k)
+      if (a[k] != 0.0f) K1 = k;
+    for (int k = 0; k < n2; ++k)
+      if (a[h1 + k] != 0.0f) K2 = std::max(K2, std::min(k, n2 - k));
+    return Support{K1, K2};
+  }
+  void compute_supports() {
+    fullsup = Support{h1 - 1, n2 / 2};
+    gsup.clear();
+    for (size_t g = 0; g < T.group_first.size(); ++g) gsup.push_back(level_support(T.items[T.group_first[g]].level));
+    lowsup = level_support(T.J);
+  }
 
   size_t lds_rows_r2c() const { return sizeof(float2) * ((size_t)n1 + 2 * (size_t)n1 * (g_rows / 2)); }
   size_t lds_rows_c2r() const {
@@ -254,8 +315,8 @@ struct surfhip_ns_plan {
     for (int ch = 0; ch < nch; ++ch) {
       if (fast) {
         const FastCtx c = ctx();
-        ops.b_fwd(c, x + ch, 0, nch, 1, VT.p);
-        ops.a_fwd(c, VT.p, X[ch].p);
+        timed(4, [&] { ops.b_fwd(c, x + ch, 0, nch, 1, VT.p, fullsup); });
+        timed(4, [&] { ops.a_fwd(c, VT.p, X[ch].p); });
         continue;
       }
       r2c(x + ch, nch, X[ch].p);
@@ -270,8 +331,9 @@ struct surfhip_ns_plan {
     for (int ch = 0; ch < nch; ++ch) {
       if (fast) {
         const FastCtx c = ctx();
-        ops.a_ana(c, X[ch].p, VT.p, item0, nw);
-        ops.b_inv(c, VT.p, nw, out + ch, (size_t)N * nch, (size_t)nch, scale);
+        const Support sp = support_of(item0);
+        timed(0, [&] { ops.a_ana(c, X[ch].p, VT.p, item0, nw, group_size(item0), sp); });
+        timed(1, [&] { ops.b_inv(c, VT.p, nw, out + ch, (size_t)N * nch, (size_t)nch, scale, sp); });
         continue;
       }
       for (int w = 0; w < nw; ++w) {
@@ -298,15 +360,18 @@ struct surfhip_ns_plan {
       return;
     }
     const FastCtx c = ctx();
-    for (int ch = 0; ch < nch; ++ch) zero(acc[ch].p);
+    for (int ch = 0; ch < nch; ++ch) timed(4, [&] { zero(acc[ch].p); });
     const int G = (int)T.group_first.size();
     for (int g = 0; g <= G; ++g) {
       const int i0 = g < G ? T.group_first[g] : 0, nw = g < G ? T.group_count[g] : 1;
       const bool last = g == G;
       for (int ch = 0; ch < nch; ++ch) {
-        ops.b_fwd(c, coeffs + (int64_t)i0 * N * nch + ch, (size_t)N * nch, (size_t)nch, nw, VT.p);
-        ops.a_syn(c, VT.p, nw, acc[ch].p, nullptr, nullptr, Vfin.p, i0, 1, last ? 1 : 0);
-        if (last) ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N);
+        const Support sp = support_of(i0);
+        timed(2, [&] {
+          ops.b_fwd(c, coeffs + (int64_t)i0 * N * nch + ch, (size_t)N * nch, (size_t)nch, nw, VT.p, sp);
+        });
+        timed(3, [&] { ops.a_syn(c, VT.p, nw, acc[ch].p, nullptr, nullptr, Vfin.p, i0, 1, last ? 1 : 0, nw, sp); });
+        if (last) timed(4, [&] { ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N, fullsup); });
       }
     }
     open_group = -2;
@@ -320,7 +385,7 @@ struct surfhip_ns_plan {
     if (open_group == -2) return;
     if (!open_fresh) {
       const FastCtx c = ctx();
-      for (int ch = 0; ch < nch; ++ch) ops.a_close(c, acc[ch].p, accg[ch].p, open_item);
+      for (int ch = 0; ch < nch; ++ch) ops.a_close(c, acc[ch].p, accg[ch].p, open_item, support_of(open_item));
     }
     open_group = -2;
     open_fresh = true;
@@ -338,8 +403,10 @@ struct surfhip_ns_plan {
       }
       const FastCtx c = ctx();
       for (int ch = 0; ch < nch; ++ch) {
-        ops.b_fwd(c, band + ch, 0, (size_t)nch, 1, VT.p);
-        ops.a_syn(c, VT.p, 1, acc[ch].p, open_fresh ? nullptr : accg[ch].p, accg[ch].p, nullptr, i, 0, 0);
+        const Support sp = support_of(i);
+        ops.b_fwd(c, band + ch, 0, (size_t)nch, 1, VT.p, sp);
+        ops.a_syn(c, VT.p, 1, acc[ch].p, open_fresh ? nullptr : accg[ch].p, accg[ch].p, nullptr, i, 0, 0,
+                  group_size(i), sp);
       }
       open_fresh = false;
       return;
@@ -358,7 +425,7 @@ struct surfhip_ns_plan {
       const FastCtx c = ctx();
       for (int ch = 0; ch < nch; ++ch) {
         ops.a_inv(c, acc[ch].p, Vfin.p);
-        ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N);
+        ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N, fullsup);
       }
       return;
     }
@@ -387,6 +454,20 @@ int pick_group(int n) {
 }
 
 }  // namespace
+
+// Run `body` with the mask view swapped for an adjoint (restored after).
+template <class F> void with_masks(surfhip_ns_plan *plan, int syn_uses_ana, int ana_uses_syn, F &&body) {
+  const NSMaskView keep = plan->dview;
+  plan->dview.syn_uses_ana = syn_uses_ana;
+  plan->dview.ana_uses_syn = ana_uses_syn;
+  try {
+    body();
+  } catch (...) {
+    plan->dview = keep;
+    throw;
+  }
+  plan->dview = keep;
+}
 
 extern "C" {
 
@@ -439,12 +520,17 @@ int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, in
     P->planes.upload(P->T.planes);
     P->ditems.upload(P->T.items);
     P->dview = NSMaskView{P->hg1.p, P->hg1c.p, P->hg2.p, P->hg2c.p, P->a.p, P->l.p, P->c.p, P->planes.p,
-                          P->h1,    n2,         n3,        P->T.J,    P->T.tight};
+                          P->h1,    n2,         n3,        P->T.J,    P->T.tight, 0, 0};
     for (int ch = 0; ch < P->nch; ++ch) {
       P->X[ch].alloc(P->H);
       P->acc[ch].alloc(P->H);
     }
     if (P->fast) {
+      P->compute_supports();
+      if (std::getenv("SURFHIP_NS_NO_PRUNE")) {
+        for (auto &g : P->gsup) g = P->fullsup;
+        P->lowsup = P->fullsup;
+      }
       for (int c : P->T.group_count) P->maxw = std::max(P->maxw, c);
       P->VT.alloc((size_t)P->maxw * P->H);
       P->Vfin.alloc(P->H);
@@ -489,6 +575,20 @@ int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x) {
   });
 }
 
+int surfhip_ns_adjoint(surfhip_ns_plan *plan, const void *coeffs, void *x) {
+  return guarded([&] {
+    if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    with_masks(plan, 1, 0, [&] { plan->synthesize((const float *)coeffs, (float *)x); });
+  });
+}
+
+int surfhip_ns_synthesize_adjoint(surfhip_ns_plan *plan, const void *x, void *coeffs) {
+  return guarded([&] {
+    if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    with_masks(plan, 0, 1, [&] { plan->analyze((const float *)x, (float *)coeffs); });
+  });
+}
+
 int surfhip_ns_stream_begin(surfhip_ns_plan *plan, const void *x) {
   return guarded([&] {
     if (!plan || !x) fail(SURFHIP_ERR_ARG, "null argument");
@@ -525,6 +625,36 @@ int surfhip_ns_stream_finish(surfhip_ns_plan *plan, void *x) {
   return guarded([&] {
     if (!plan || !x) fail(SURFHIP_ERR_ARG, "null argument");
     plan->finish((float *)x);
+  });
+}
+
+int surfhip_ns_set_timing(surfhip_ns_plan *plan, int32_t enable) {
+  return guarded([&] {
+    if (!plan) fail(SURFHIP_ERR_ARG, "null plan");
+    plan->collect_timing();
+    for (int c = 0; c < SURFHIP_NS_TIMING_CLASSES; ++c) {
+      plan->tcount[c] = 0;
+      plan->tms[c] = 0.0;
+    }
+    plan->timing = enable != 0;
+  });
+}
+
+int surfhip_ns_timing(surfhip_ns_plan *plan, int64_t *launches, double *ms, char *names, size_t n) {
+  return guarded([&] {
+    if (!plan) fail(SURFHIP_ERR_ARG, "null plan");
+    plan->collect_timing();
+    for (int c = 0; c < SURFHIP_NS_TIMING_CLASSES; ++c) {
+      if (launches) launches[c] = plan->tcount[c];
+      if (ms) ms[c] = plan->tms[c];
+    }
+    const char *s = plan->fast ? "pencil_analysis,plane_analysis,plane_synthesis,pencil_synthesis,other"
+                               : "unclassified,unclassified,unclassified,unclassified,unclassified";
+    if (names && n) {
+      size_t m = std::min(n - 1, std::strlen(s));
+      std::memcpy(names, s, m);
+      names[m] = '\0';
+    }
   });
 }
 

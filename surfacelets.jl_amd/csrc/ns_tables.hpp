@@ -37,6 +37,12 @@ struct NSMaskView {
   const float *c;           // J x (h1+n2+n3): 1 - l_j per axis (formed in FP64)
   const float *planes;      // wedge plane pool (sqrt'd when tight), each [lower axis][higher axis]
   int32_t h1, n2, n3, J, tight;
+  // Mask swaps for the adjoints: every subband operator irfft . M . rfft is
+  // self-adjoint (M real and even), so A^T is the accumulation with the
+  // analysis masks and S^T the analysis with the synthesis masks.  Only a
+  // bank that is not tight has different analysis and synthesis masks.
+  int32_t syn_uses_ana = 0;  // "synthesis" masks are the analysis masks
+  int32_t ana_uses_syn = 0;  // "analysis" masks are the synthesis masks
 };
 
 SH_HD inline float ns_plane(const NSMaskView &v, int32_t off, int32_t pair, int k1, int k2, int k3) {
@@ -73,7 +79,7 @@ SH_HD inline float ns_group(const NSMaskView &v, const NSItem &it, int k1, int k
     return a[k1] * a[v.h1 + k2] * a[v.h1 + v.n2 + k3];
   }
   float B = ns_bandpass(v, it.level, k1, k2, k3);
-  if (!analysis && !v.tight) return B;
+  if (!v.tight && (analysis ? v.ana_uses_syn : !v.syn_uses_ana)) return B;
   size_t p3 = ((size_t)k1 * v.n2 + (size_t)k2) * v.n3 + k3;
   size_t p2 = (size_t)k2 * v.n3 + k3;
   float h = it.hourglass == 1 ? v.hg1[p3] : v.hg1c[p3] * (it.hourglass == 2 ? v.hg2[p2] : v.hg2c[p2]);
@@ -83,7 +89,7 @@ SH_HD inline float ns_group(const NSMaskView &v, const NSItem &it, int k1, int k
 // Wedge factor w_a w_b (src/transforms/ns_surfacelet.jl:376-427); 1 for the
 // lowpass and for the synthesis of a bank that is not tight (unit wedge).
 SH_HD inline float ns_wedge(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
-  if (it.lowpass || (!analysis && !v.tight)) return 1.0f;
+  if (it.lowpass || (!v.tight && (analysis ? v.ana_uses_syn : !v.syn_uses_ana))) return 1.0f;
   return ns_plane(v, it.pa_off, it.pa_pair, k1, k2, k3) * ns_plane(v, it.pb_off, it.pb_pair, k1, k2, k3);
 }
 
@@ -102,7 +108,7 @@ struct NSTables {
   std::vector<int32_t> group_first, group_count;  // items of each group (subband ranges)
   NSMaskView view() const {
     return NSMaskView{hg1.data(), hg1c.data(), hg2.data(), hg2c.data(), a.data(), l.data(),
-                      c.data(), planes.data(), h1, n2, n3, J, tight};
+                      c.data(), planes.data(), h1, n2, n3, J, tight, 0, 0};
   }
 };
 

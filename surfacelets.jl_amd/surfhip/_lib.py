@@ -62,12 +62,17 @@ _SIGS = {
     "surfhip_ns_set_stream": (ctypes.c_int, [_P, _P]),
     "surfhip_ns_analyze": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_ns_synthesize": (ctypes.c_int, [_P, _P, _P]),
+    "surfhip_ns_adjoint": (ctypes.c_int, [_P, _P, _P]),
+    "surfhip_ns_synthesize_adjoint": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_ns_stream_begin": (ctypes.c_int, [_P, _P]),
     "surfhip_ns_stream_subband": (ctypes.c_int, [_P, _I, _P]),
     "surfhip_ns_stream_inverse_begin": (ctypes.c_int, [_P]),
     "surfhip_ns_stream_accumulate": (ctypes.c_int, [_P, _I, _P]),
     "surfhip_ns_stream_finish": (ctypes.c_int, [_P, _P]),
     "surfhip_ns_mask_host": (ctypes.c_int, [ctypes.POINTER(NSParams), _I, _I, _I, _I, ctypes.POINTER(ctypes.c_float)]),
+    "surfhip_ns_set_timing": (ctypes.c_int, [_P, _I]),
+    "surfhip_ns_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.c_char_p, ctypes.c_size_t]),
     "surfhip_s_check_size": (ctypes.c_int, [ctypes.POINTER(SParams), _I, _I, _I]),
     "surfhip_s_counts": (ctypes.c_int, [ctypes.POINTER(SParams), _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_int64)]),
     "surfhip_s_layout": (ctypes.c_int, [ctypes.POINTER(SParams), _I, _I, _I, ctypes.POINTER(ctypes.c_int64),
@@ -77,6 +82,7 @@ _SIGS = {
     "surfhip_s_set_stream": (ctypes.c_int, [_P, _P]),
     "surfhip_s_analyze": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_s_synthesize": (ctypes.c_int, [_P, _P, _P]),
+    "surfhip_s_adjoint": (ctypes.c_int, [_P, _P, _P]),
 }
 
 _lock = threading.Lock()

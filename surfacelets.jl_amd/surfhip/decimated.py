@@ -382,6 +382,19 @@ def synthesize(t, c, *, workspace=None, threaded=None):
     return synthesize_(x, t, c, workspace=workspace, threaded=threaded)
 
 
+def adjoint(t, c, *, workspace=None):
+    """x = A^T c, the adjoint of analyze (not its inverse: the decimated
+    transform is a frame but not a tight one, src/transforms/surfacelet.jl:24-26)."""
+    t, workspace = _split(t, workspace)
+    dims = workspace.dims if workspace is not None else c.layout.signal_size
+    ws = _workspace_for(workspace, t, c.dtype, dims, c.device)
+    _check_coefficients(c, ws)
+    x = empty_colmajor(ws.dims, ws.dtype, ws.device)
+    ws.bind_stream()
+    _lib.call("surfhip_s_adjoint", ws._plan, ptr(c.data), ptr(x))
+    return x
+
+
 def check_size(t, dims):
     """Raise the reference's ArgumentError (with the nearest fitting length)
     when `dims` does not fit `t` (src/transforms/surfacelet.jl:253-269)."""

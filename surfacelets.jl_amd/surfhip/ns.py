@@ -165,6 +165,20 @@ class NSSurfaceletWorkspace:
     def bind_stream(self):
         _lib.call("surfhip_ns_set_stream", self._plan, ctypes_void(torch.cuda.current_stream(self.device).cuda_stream))
 
+    def set_timing(self, enable):
+        """Bracket every kernel launch with HIP events (a benchmarking hook)."""
+        _lib.call("surfhip_ns_set_timing", self._plan, 1 if enable else 0)
+
+    def timing(self):
+        """{kernel class: (launches, total ms)} since timing was switched on."""
+        n = 5
+        launches = (ctypes.c_int64 * n)()
+        ms = (ctypes.c_double * n)()
+        names = ctypes.create_string_buffer(256)
+        _lib.call("surfhip_ns_timing", self._plan, launches, ms, names, 256)
+        keys = names.value.decode().split(",")
+        return {k: (int(launches[i]), float(ms[i])) for i, k in enumerate(keys)}
+
     def band_buffer(self):
         if self._band is None:
             self._band = empty_colmajor(self.dims, self.dtype, self.device)
@@ -379,3 +393,32 @@ def ns_mask_host(t, dims, i):
     _lib.call("surfhip_ns_mask_host", ctypes.byref(t.c_params()), dims[0], dims[1], dims[2], i,
               out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
     return out.reshape(h, order="F")
+
+
+def adjoint(t, c, *, workspace=None):
+    """x = A^T c, the adjoint of analyze (synthesize with the analysis masks;
+    equal to synthesize for a tight bank)."""
+    t, workspace = _split(t, workspace)
+    check_device(c, "coefficient array")
+    dims = _check_volume(c.shape[:3])
+    ws = workspace if workspace is not None else make_workspace(t, _eltype(c), dims, prewarm=False, device=c.device)
+    check_workspace(ws, t, _eltype(c), dims)
+    _check_coefficients(c, t, ws.dims)
+    cin = as_colmajor(c if c.dtype == ws.dtype else c.to(ws.dtype))
+    x = empty_colmajor(ws.dims, ws.dtype, ws.device)
+    ws.bind_stream()
+    _lib.call("surfhip_ns_adjoint", ws.handle, ptr(cin), ptr(x))
+    return x
+
+
+def synthesize_adjoint(t, x, *, workspace=None):
+    """c = S^T x, the adjoint of synthesize (analyze with the synthesis masks;
+    equal to analyze for a tight bank)."""
+    t, workspace = _split(t, workspace)
+    check_device(x, "volume")
+    ws = _workspace_for(workspace, t, x)
+    xin = _input(x, ws)
+    c = empty_colmajor(ws.dims + (ws.nsub,), ws.dtype, ws.device)
+    ws.bind_stream()
+    _lib.call("surfhip_ns_synthesize_adjoint", ws.handle, ptr(xin), ptr(c))
+    return c

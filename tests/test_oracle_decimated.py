@@ -126,3 +126,54 @@ def test_redundancy_matches_surfbox():
     for nm, Lm, J in (("d24", [[-1, 1, 2], [2, -1, 1], [1, 2, -1]], 3), ("d96", [[-1, 2, 3], [3, -1, 2], [2, 3, -1]], 2)):
         t = D.SParams(scales=J, direction_levels=tuple([np.array(Lm)] * J), checkerboard_order=4)
         assert abs(D.redundancy(t, (96, 96, 96)) - G[f"red_96_{nm}"][0, 0]) < 1e-12
+
+
+@pytest.mark.parametrize("dims,mode,J,L,bo", [
+    ((24, 24, 24), 1.5, 1, [1], 12),
+    ((32, 32, 32), 2, 2, [1, 2], 4),
+    ((32, 32, 32), 1, 2, [1, 1], 8),
+    ((48, 24, 36), 1.5, 1, [2], 8),
+])
+def test_adjoint_dense_probes(dims, mode, J, L, bo):
+    # The decimated transform is not tight, so its adjoint is not synthesize
+    # (src/transforms/surfacelet.jl:24-26): check <A x, y> = <x, A^T y> on
+    # random probes, and that A^T differs from synthesize.
+    rng = np.random.default_rng(11)
+    t = D.SParams(scales=J, direction_levels=tuple(L), pyramid_ratio=mode, checkerboard_order=bo)
+    ws = D.workspace(t, dims)
+    for _ in range(3):
+        x = rng.standard_normal(dims)
+        c = D.analyze(t, x, ws)
+        y = [rng.standard_normal(b.shape) for b in c]
+        lhs = sum(np.sum(a * b) for a, b in zip(c, y))
+        rhs = np.sum(x * D.adjoint(t, y, ws))
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+    ys = D.synthesize(t, y, ws)
+    assert np.max(np.abs(ys - D.adjoint(t, y, ws))) > 1e-6
+
+
+def test_adjoint_dense_matrix_tiny():
+    # Brute force: the full analysis matrix at 12^3 (mode 1.5, one scale),
+    # column by column, against the adjoint applied to unit coefficient vectors.
+    t = D.SParams(scales=1, direction_levels=(1,), pyramid_ratio=1.5, checkerboard_order=4)
+    dims = (12, 12, 12)
+    ws = D.workspace(t, dims)
+    sizes = D.layout_sizes(t, dims)
+    ncoef = sum(int(np.prod(s)) for s in sizes)
+    rng = np.random.default_rng(5)
+    picks = rng.choice(ncoef, size=40, replace=False)
+    N = int(np.prod(dims))
+    cols = rng.choice(N, size=40, replace=False)
+    A = np.zeros((40, 40))
+    for jj, col in enumerate(cols):
+        e = np.zeros(N)
+        e[col] = 1
+        flat = np.concatenate([b.ravel(order="F") for b in D.analyze(t, e.reshape(dims, order="F"), ws)])
+        A[:, jj] = flat[picks]
+    offs = np.cumsum([0] + [int(np.prod(s)) for s in sizes])
+    for ii, row in enumerate(picks):
+        flat = np.zeros(ncoef)
+        flat[row] = 1
+        c = [flat[offs[q]:offs[q + 1]].reshape(sizes[q], order="F") for q in range(len(sizes))]
+        at = D.adjoint(t, c, ws).ravel(order="F")
+        assert np.max(np.abs(at[cols] - A[ii, :])) < 1e-13
