@@ -7,13 +7,13 @@ bernstein_order = 12, halfband_order = 1, transition = 1, tight) on a
 workspace; the input and the coefficients stay resident in HBM.
 
 metric: voxel-subbands per second = n1 n2 n3 S / (seconds per step).
-roofline: the dominant kernel class, timed live with HIP events that the
-library records around each of its launches on the stream they run on
-(a second, instrumented loop; the headline ms_per_step comes from an
-uninstrumented loop).  The classes that touch the user's data carry the
-algorithmic bytes: the analysis plane pass writes the S subbands (4 N bytes
-each) and the synthesis plane pass reads them; achieved = those bytes over
-the class's summed launch time, against 8 TB/s.  The whole step's
+roofline: the slower of the two plane kernels, timed live with HIP events
+that the library records around each of its launches on the stream they run
+on (a second, instrumented loop; the headline ms_per_step comes from an
+uninstrumented loop).  The plane kernels are the ones that touch the user's
+data: the inverse plane pass writes the S subbands and the volume (4 N (S+1)
+bytes per step), the forward plane pass reads them; achieved = those bytes
+over the kernel's summed launch time, against 8 TB/s.  The whole step's
 algorithmic bytes, 8 N (S + 1), over ms_per_step are reported beside it.
 cpu_baseline: the FP32 CPU restatement of the reference's schedule
 (oracle/cpu_schedule.py) on the host cores, on a bounded sample.
@@ -141,8 +141,10 @@ def main():
     cls = ws.timing()
     ws.set_timing(False)
     kernels = {k: {"launches_per_step": n / args.steps, "ms_per_step": t / args.steps} for k, (n, t) in cls.items()}
-    # Algorithmic HBM bytes per step of each class (the user's data only).
-    alg = {"plane_analysis": 4 * N * S, "plane_synthesis": 4 * N * S}
+    # Algorithmic HBM bytes per step of the two plane kernels, the ones that
+    # touch the user's data: the inverse plane pass writes the S subbands and
+    # the volume, the forward plane pass reads the volume and the S subbands.
+    alg = {"plane_inverse": 4 * N * (S + 1), "plane_forward": 4 * N * (S + 1)}
     dom = max(cls, key=lambda k: cls[k][1])
     dom_plane = max(alg, key=lambda k: cls.get(k, (0, 0.0))[1])
     launches, tms = cls[dom_plane]

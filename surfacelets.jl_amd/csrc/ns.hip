@@ -22,7 +22,7 @@
 
 #include "fft.hpp"
 #include "hip_util.hpp"
-#include "ns_fast.hpp"
+#include "ns_fused.hpp"
 #include "ns_tables.hpp"
 
 namespace surfhip {
@@ -185,9 +185,9 @@ __global__ void k_zero(float2 *__restrict__ p, int64_t n) {
 
 using namespace surfhip;
 
-// A plan runs one of two kernel sets on the same tables: the fast fused path
-// (ns_fast.hip) on power-of-two grids it covers, or the general three-pass
-// path above on any other size.  Both evaluate the same mask expression.
+// A plan runs one of two kernel sets on the same tables: the factored path
+// (ns_fused.hip) on the power-of-two grids it covers, or the general
+// three-pass path above on any other size.  Both evaluate the same masks.
 struct surfhip_ns_plan {
   NSTables T;
   int dtype = SURFHIP_F32;
@@ -197,24 +197,41 @@ struct surfhip_ns_plan {
   int n1 = 0, n2 = 0, n3 = 0, h1 = 0;
   FftPlan1D f1{}, f2{}, f3{};
   int g_rows = 16, g_axis = 16;
-  DevBuf<float> hg1, hg1c, hg2, hg2c, a, l, c, planes;
+  DevBuf<float> gana, gsyn, planes;
   DevBuf<float2> tw, X[2], acc[2], work;
   DevBuf<NSItem> ditems;
   NSMaskView dview{};
   bool streaming_fwd = false;
-  // fast path
-  bool fast = false;
-  FastOps ops;
-  int maxw = 1;                  // largest group
-  DevBuf<float2> VT, Vfin, accg[2];
-  // optional per-class event timing
+
+  // ---- factored path
+  bool fused = false;
+  FusedOps fops;
+  int nslabs = 0;
+  std::vector<GroupDev> groups;     // 3 J groups, then the lowpass
+  std::vector<int> slab_group;      // group of each slab
+  std::vector<uint32_t> live;       // nslabs x h1 tile words
+  std::vector<int4> units;          // analysis pencil units, group-major, AX3 groups first
+  std::vector<int> ubeg, uend;      // units of each group
+  int ax_ubeg[2] = {0, 0}, ax_uend[2] = {0, 0};
+  std::vector<PlaneJob> jobs;       // [analysis AX3, AX2][synthesis AX3, AX2][whole grid][S fresh][S continuing]
+  int ja_ax[2] = {0, 0}, jna_ax[2] = {0, 0}, js_ax[2] = {0, 0}, jns_ax[2] = {0, 0};
+  int j_full = 0, j_fresh = 0, j_cont = 0;
+  std::vector<int32_t> glist;       // synthesis group order: AX2 groups, then AX3 groups + lowpass
+  int gl_beg[2] = {0, 0}, gl_n[2] = {0, 0};
+  DevBuf<GroupDev> dgroups;
+  DevBuf<uint32_t> dlive;
+  DevBuf<int4> dunits;
+  DevBuf<PlaneJob> djobs;
+  DevBuf<int32_t> dglist;
+  DevBuf<float2> U[2], R[2], TV, accf;
+  int cached_group = -1;            // streaming analysis: the group whose slabs U holds
+  std::vector<char> touched;        // streaming synthesis: slabs holding a partial sum
+
+  // ---- optional per-class event timing
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> tev;
   int64_t tcount[SURFHIP_NS_TIMING_CLASSES] = {0};
   double tms[SURFHIP_NS_TIMING_CLASSES] = {0};
-  int open_group = -2;           // streaming synthesis: group being accumulated (-2: none)
-  int open_item = 0;             // an item of that group
-  bool open_fresh = true;        // no term accumulated yet in the open group
 
   // Run `launch` bracketed by events of class `cls` when timing is on.
   template <class F> void timed(int cls, F &&launch) {
@@ -250,30 +267,183 @@ struct surfhip_ns_plan {
     }
   }
 
-  FastCtx ctx() const {
-    return FastCtx{stream, n1, n2, n3, h1, H, dview, ditems.p, tw.p + f1.tw, tw.p + f2.tw, tw.p + f3.tw};
+  FusedCtx ctx() const {
+    return FusedCtx{stream, n1, n2, n3, h1, H, dview, ditems.p, dgroups.p, dlive.p,
+                    tw.p + f1.tw, tw.p + f2.tw, tw.p + f3.tw};
   }
-  int group_of(int i) const { return T.items[i].lowpass ? -1 : T.items[i].group; }
-  int group_size(int i) const { return T.items[i].lowpass ? 1 : T.group_count[T.items[i].group]; }
-  // Frequency support of each group (and the lowpass): see Support.
-  std::vector<Support> gsup;
-  Support lowsup{}, fullsup{};
-  Support support_of(int i) const { return T.items[i].lowpass ? lowsup : gsup[T.items[i].group]; }
-  Support level_support(int lev) const {
-    const int L = h1 + n2 + n3;
-    const float *a = T.a.data() + (size_t)lev * L;
-    int K1 = 0, K2 = 0;
-    for (int k = 0; k < h1; ++k)
-      if (a[k] != 0.0f) K1 = k;
-    for (int k = 0; k < n2; ++k)
-      if (a[h1 + k] != 0.0f) K2 = std::max(K2, std::min(k, n2 - k));
-    return Support{K1, K2};
-  }
-  void compute_supports() {
-    fullsup = Support{h1 - 1, n2 / 2};
-    gsup.clear();
-    for (size_t g = 0; g < T.group_first.size(); ++g) gsup.push_back(level_support(T.items[T.group_first[g]].level));
-    lowsup = level_support(T.J);
+  int group_of(int i) const { return T.items[i].lowpass ? (int)groups.size() - 1 : T.items[i].group; }
+
+  // ---- factored-path set-up: groups, slabs, live tiles, units and jobs.
+  void build_fused() {
+    const int G = (int)T.group_first.size();
+    const int TW = fops.tile;
+    groups.clear();
+    slab_group.clear();
+    for (int g = 0; g <= G; ++g) {
+      GroupDev gd{};
+      if (g < G) {
+        gd.first = T.group_first[g];
+        gd.nA = T.group_na[g];
+        gd.nB = T.group_nb[g];
+        gd.orient = T.items[gd.first].hourglass == 3 ? AX2 : AX3;
+        gd.lowpass = 0;
+      } else {
+        gd = GroupDev{0, 1, 1, AX3, 0, 1};
+      }
+      gd.slab0 = (int)slab_group.size();
+      for (int q = 0; q < gd.nB; ++q) slab_group.push_back(g);
+      groups.push_back(gd);
+    }
+    nslabs = (int)slab_group.size();
+    // Live tiles: tile t of row k1 of slab (g, q) holds a nonzero iff some bin
+    // of it has a nonzero group factor and b-plane, for the analysis or the
+    // synthesis masks.  Exact zeros of the tables decide it, so skipping dead
+    // tiles changes no result.
+    const NSMaskView hv = T.view();
+    const bool noprune = std::getenv("SURFHIP_NS_NO_PRUNE") != nullptr;
+    live.assign((size_t)nslabs * h1, 0u);
+    const size_t V = (size_t)H;
+    for (int g = 0; g <= G; ++g) {
+      const GroupDev &gd = groups[g];
+      const bool ax3 = gd.orient == AX3;
+      const int nt = ax3 ? n2 : n3, np = ax3 ? n3 : n2, ntiles = nt / TW;
+      const uint32_t all = ntiles >= 32 ? 0xffffffffu : ((1u << ntiles) - 1u);
+      const float *ga = hv.gana + (size_t)g * V;
+      const float *gs = hv.gsyn + (size_t)g * V;
+      for (int q = 0; q < gd.nB; ++q) {
+        const NSItem &it = T.items[gd.first + q];
+        uint32_t *lw = live.data() + (size_t)(gd.slab0 + q) * h1;
+        for (int k1 = 0; k1 < h1; ++k1) {
+          if (noprune) {
+            lw[k1] = all;
+            continue;
+          }
+          uint32_t w = 0;
+          for (int kt = 0; kt < nt; ++kt) {
+            const int t = kt / TW;
+            if ((w >> t) & 1u) continue;
+            for (int kp = 0; kp < np; ++kp) {
+              const int k2 = ax3 ? kt : kp, k3 = ax3 ? kp : kt;
+              const size_t q3 = ((size_t)k1 * n2 + k2) * n3 + k3;
+              const float wb = it.lowpass ? 1.0f : ns_plane(hv, it.pb_off, it.pb_pair, k1, k2, k3);
+              const bool la = ga[q3] != 0.0f && wb != 0.0f;
+              const bool ls = gs[q3] != 0.0f && (!T.tight || wb != 0.0f);
+              if (la || ls) {
+                w |= 1u << t;
+                break;
+              }
+            }
+          }
+          lw[k1] = w;
+        }
+      }
+    }
+    // Analysis units, AX3 groups first.
+    units.clear();
+    ubeg.assign(G + 1, 0);
+    uend.assign(G + 1, 0);
+    for (int o = 0; o < 2; ++o) {
+      ax_ubeg[o] = (int)units.size();
+      for (int g = 0; g <= G; ++g) {
+        const GroupDev &gd = groups[g];
+        if (gd.orient != o) continue;
+        const int ntiles = (o == AX3 ? n2 : n3) / TW;
+        ubeg[g] = (int)units.size();
+        for (int k1 = 0; k1 < h1; ++k1)
+          for (int t = 0; t < ntiles; ++t) {
+            bool any = false;
+            for (int q = 0; q < gd.nB; ++q) any |= ((live[(size_t)(gd.slab0 + q) * h1 + k1] >> t) & 1u) != 0;
+            if (any) units.push_back(make_int4(g, k1, t * TW, 0));
+          }
+        uend[g] = (int)units.size();
+      }
+      ax_uend[o] = (int)units.size();
+    }
+    // Plane job of slab (g, q), a-indices [p0, p1): plane a of item (p, q)
+    // sits at pa0 + (p - p0) pa_step in the pool (the planes of one wedge
+    // level are stored back to back).
+    auto mkjob = [&](int g, int q, int p0, int p1, int init, int out0, int step) {
+      const GroupDev &gd = groups[g];
+      PlaneJob jb{};
+      jb.slab = gd.slab0 + q;
+      jb.group = g;
+      jb.q = q;
+      jb.p0 = p0;
+      jb.p1 = p1;
+      jb.init = init;
+      jb.out0 = out0;
+      jb.out_step = step;
+      const int first = gd.first;
+      jb.pa0 = gd.lowpass ? -1 : T.items[first + p0 * gd.nB + q].pa_off;
+      jb.pa_step = (gd.lowpass || gd.nA < 2) ? 0 : T.items[first + gd.nB + q].pa_off - T.items[first + q].pa_off;
+      for (int p = p0; p < p1 && !gd.lowpass; ++p)
+        if (T.items[first + p * gd.nB + q].pa_off != (jb.pa0 < 0 ? -1 : jb.pa0 + (p - p0) * jb.pa_step))
+          fail(SURFHIP_ERR_ARG, "internal: wedge planes of a level are not evenly spaced");
+      return jb;
+    };
+    // Plane jobs.  Analysis: one job per subband (the plane blocks of one
+    // slab plane then run on one XCD, close in time, and share its reads in
+    // L2).  Synthesis: one job per slab, summing its nA subbands in order.
+    jobs.clear();
+    for (int o = 0; o < 2; ++o) {
+      ja_ax[o] = (int)jobs.size();
+      for (int g = 0; g <= G; ++g) {
+        const GroupDev &gd = groups[g];
+        if (gd.orient != o) continue;
+        for (int q = 0; q < gd.nB; ++q)
+          for (int p = 0; p < gd.nA; ++p)
+            jobs.push_back(mkjob(g, q, p, p + 1, 1, gd.first + p * gd.nB + q, 0));
+      }
+      jna_ax[o] = (int)jobs.size() - ja_ax[o];
+    }
+    for (int o = 0; o < 2; ++o) {
+      js_ax[o] = (int)jobs.size();
+      for (int g = 0; g <= G; ++g) {
+        const GroupDev &gd = groups[g];
+        if (gd.orient != o) continue;
+        for (int q = 0; q < gd.nB; ++q)
+          jobs.push_back(mkjob(g, q, 0, gd.nA, 1, gd.first + q, gd.nB));
+      }
+      jns_ax[o] = (int)jobs.size() - js_ax[o];
+    }
+    j_full = (int)jobs.size();
+    {
+      PlaneJob full{};
+      full.group = -1;
+      full.p1 = 1;
+      full.init = 1;
+      full.pa0 = -1;
+      jobs.push_back(full);
+    }
+    for (int fresh = 1; fresh >= 0; --fresh) {
+      (fresh ? j_fresh : j_cont) = (int)jobs.size();
+      for (int s = 0; s < T.S; ++s) {
+        const int g = group_of(s);
+        const GroupDev &gd = groups[g];
+        const int w = s - gd.first, p = w / gd.nB, q = w % gd.nB;
+        jobs.push_back(mkjob(g, q, p, p + 1, fresh, 0, 0));
+      }
+    }
+    // Synthesis group order.
+    glist.clear();
+    for (int o : {AX2, AX3}) {
+      gl_beg[o] = (int)glist.size();
+      for (int g = 0; g <= G; ++g)
+        if (groups[g].orient == o) glist.push_back(g);
+      gl_n[o] = (int)glist.size() - gl_beg[o];
+    }
+    dgroups.upload(groups);
+    dlive.upload(live);
+    dunits.upload(units);
+    djobs.upload(jobs);
+    dglist.upload(glist);
+    for (int ch = 0; ch < nch; ++ch) {
+      U[ch].alloc((size_t)nslabs * H);
+      R[ch].alloc((size_t)nslabs * H);
+    }
+    TV.alloc(H);
+    accf.alloc(H);
+    touched.assign(nslabs, 0);
   }
 
   size_t lds_rows_r2c() const { return sizeof(float2) * ((size_t)n1 + 2 * (size_t)n1 * (g_rows / 2)); }
@@ -304,19 +474,20 @@ struct surfhip_ns_plan {
                        n3, g_axis, pl, tw.p, dview, it);
     SH_LAUNCHED();
   }
-  void zero(float2 *p) {
-    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(NT), 0, stream, p, H);
+  void zero(float2 *p, int64_t n) {
+    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(NT), 0, stream, p, n);
     SH_LAUNCHED();
   }
 
   // X[ch] = rfft(channel ch of x)
   void forward_spectrum(const float *x) {
     static const NSItem none{};
+    cached_group = -1;
     for (int ch = 0; ch < nch; ++ch) {
-      if (fast) {
-        const FastCtx c = ctx();
-        timed(4, [&] { ops.b_fwd(c, x + ch, 0, nch, 1, VT.p, fullsup); });
-        timed(4, [&] { ops.a_fwd(c, VT.p, X[ch].p); });
+      if (fused) {
+        const FusedCtx c = ctx();
+        timed(2, [&] { fops.q_syn[AX3](c, AX3, x + ch, 0, nch, TV.p, djobs.p + j_full, 1); });
+        timed(4, [&] { fops.p_fwd(c, TV.p, X[ch].p); });
         continue;
       }
       r2c(x + ch, nch, X[ch].p);
@@ -324,91 +495,104 @@ struct surfhip_ns_plan {
       axis<3, -1, 0>(X[ch].p, X[ch].p, none);
     }
   }
-  // Subbands item0 .. item0+nw-1 (one group) into out (subband stride N nch).
-  void subbands(int item0, int nw, float *out) {
+  // Generic path: subbands item0 .. item0+nw-1 into out (subband stride N nch).
+  void subbands_generic(int item0, int nw, float *out) {
     static const NSItem none{};
     const float scale = 1.0f / (float)N;
-    for (int ch = 0; ch < nch; ++ch) {
-      if (fast) {
-        const FastCtx c = ctx();
-        const Support sp = support_of(item0);
-        timed(0, [&] { ops.a_ana(c, X[ch].p, VT.p, item0, nw, group_size(item0), sp); });
-        timed(1, [&] { ops.b_inv(c, VT.p, nw, out + ch, (size_t)N * nch, (size_t)nch, scale, sp); });
-        continue;
-      }
+    for (int ch = 0; ch < nch; ++ch)
       for (int w = 0; w < nw; ++w) {
         axis<3, 1, 1>(X[ch].p, work.p, T.items[item0 + w]);
         axis<2, 1, 0>(work.p, work.p, none);
         c2r(work.p, out + (int64_t)w * N * nch + ch, nch, scale);
       }
-    }
   }
   void analyze(const float *x, float *coeffs) {
     forward_spectrum(x);
-    for (size_t g = 0; g < T.group_first.size(); ++g) {
-      const int i0 = T.group_first[g], nw = T.group_count[g];
-      subbands(i0, nw, coeffs + (int64_t)i0 * N * nch);
+    if (!fused) {
+      for (size_t g = 0; g < T.group_first.size(); ++g) {
+        const int i0 = T.group_first[g], nw = T.group_count[g];
+        subbands_generic(i0, nw, coeffs + (int64_t)i0 * N * nch);
+      }
+      subbands_generic(0, 1, coeffs);
+      return;
     }
-    subbands(0, 1, coeffs);
+    const FusedCtx c = ctx();
+    for (int ch = 0; ch < nch; ++ch) {
+      for (int o = 0; o < 2; ++o)
+        timed(0, [&] {
+          fops.p_ana[o](c, X[ch].p, U[ch].p, dunits.p + ax_ubeg[o], ax_uend[o] - ax_ubeg[o], o);
+        });
+      for (int o = 0; o < 2; ++o)
+        timed(1, [&] {
+          fops.q_ana[o](c, o, U[ch].p, djobs.p + ja_ax[o], jna_ax[o], coeffs + ch, N * nch, nch, 1.0f / (float)N);
+        });
+    }
+    cached_group = -1;
+  }
+  // Streaming analysis: subband i (0-based) into band.
+  void stream_subband(int i, float *band) {
+    if (!fused) {
+      subbands_generic(i, 1, band);
+      return;
+    }
+    const FusedCtx c = ctx();
+    const int g = group_of(i);
+    const int o = groups[g].orient;
+    if (cached_group != g) {
+      for (int ch = 0; ch < nch; ++ch)
+        timed(0, [&] { fops.p_ana[o](c, X[ch].p, U[ch].p, dunits.p + ubeg[g], uend[g] - ubeg[g], o); });
+      cached_group = g;
+    }
+    for (int ch = 0; ch < nch; ++ch)
+      timed(1, [&] { fops.q_ana[o](c, o, U[ch].p, djobs.p + j_fresh + i, 1, band + ch, 0, nch, 1.0f / (float)N); });
+  }
+  // Close the synthesis sum of channel ch from its slabs and write the volume.
+  void finish_channel(int ch, float *x) {
+    const FusedCtx c = ctx();
+    const bool has2 = gl_n[AX2] > 0;
+    if (has2)
+      timed(3, [&] {
+        fops.p_syn[AX2](c, AX2, R[ch].p, dglist.p + gl_beg[AX2], gl_n[AX2], accf.p, 0, nullptr);
+      });
+    timed(3, [&] {
+      fops.p_syn[AX3](c, AX3, R[ch].p, dglist.p + gl_beg[AX3], gl_n[AX3], accf.p, has2 ? 1 : 0, TV.p);
+    });
+    timed(1, [&] { fops.q_ana[AX3](c, AX3, TV.p, djobs.p + j_full, 1, x + ch, 0, nch, 1.0f / (float)N); });
   }
   void synthesize(const float *coeffs, float *x) {
-    if (!fast) {
+    if (!fused) {
       inverse_begin();
       for (int i = 1; i < T.S; ++i) accumulate(i, coeffs + (int64_t)i * N * nch);
       accumulate(0, coeffs);
       finish(x);
       return;
     }
-    const FastCtx c = ctx();
-    for (int ch = 0; ch < nch; ++ch) timed(4, [&] { zero(acc[ch].p); });
-    const int G = (int)T.group_first.size();
-    for (int g = 0; g <= G; ++g) {
-      const int i0 = g < G ? T.group_first[g] : 0, nw = g < G ? T.group_count[g] : 1;
-      const bool last = g == G;
-      for (int ch = 0; ch < nch; ++ch) {
-        const Support sp = support_of(i0);
-        timed(2, [&] {
-          ops.b_fwd(c, coeffs + (int64_t)i0 * N * nch + ch, (size_t)N * nch, (size_t)nch, nw, VT.p, sp);
-        });
-        timed(3, [&] { ops.a_syn(c, VT.p, nw, acc[ch].p, nullptr, nullptr, Vfin.p, i0, 1, last ? 1 : 0, nw, sp); });
-        if (last) timed(4, [&] { ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N, fullsup); });
-      }
+    const FusedCtx c = ctx();
+    for (int ch = 0; ch < nch; ++ch) {
+      for (int o = 0; o < 2; ++o)
+        timed(2, [&] { fops.q_syn[o](c, o, coeffs + ch, N * nch, nch, R[ch].p, djobs.p + js_ax[o], jns_ax[o]); });
+      finish_channel(ch, x);
     }
-    open_group = -2;
   }
   void inverse_begin() {
-    for (int ch = 0; ch < nch; ++ch) zero(acc[ch].p);
-    open_group = -2;
-    open_fresh = true;
-  }
-  void close_open() {
-    if (open_group == -2) return;
-    if (!open_fresh) {
-      const FastCtx c = ctx();
-      for (int ch = 0; ch < nch; ++ch) ops.a_close(c, acc[ch].p, accg[ch].p, open_item, support_of(open_item));
+    if (fused) {
+      std::fill(touched.begin(), touched.end(), 0);
+      return;
     }
-    open_group = -2;
-    open_fresh = true;
+    for (int ch = 0; ch < nch; ++ch) zero(acc[ch].p, H);
   }
   // acc += rfft(band) . M_i
   void accumulate(int i, const float *band) {
     static const NSItem none{};
-    if (fast) {
+    if (fused) {
+      const FusedCtx c = ctx();
       const int g = group_of(i);
-      if (g != open_group) {
-        close_open();
-        open_group = g;
-        open_item = i;
-        open_fresh = true;
-      }
-      const FastCtx c = ctx();
-      for (int ch = 0; ch < nch; ++ch) {
-        const Support sp = support_of(i);
-        ops.b_fwd(c, band + ch, 0, (size_t)nch, 1, VT.p, sp);
-        ops.a_syn(c, VT.p, 1, acc[ch].p, open_fresh ? nullptr : accg[ch].p, accg[ch].p, nullptr, i, 0, 0,
-                  group_size(i), sp);
-      }
-      open_fresh = false;
+      const int o = groups[g].orient;
+      const int u = groups[g].slab0 + (i - groups[g].first) % groups[g].nB;
+      const int jb = (touched[u] ? j_cont : j_fresh) + i;
+      touched[u] = 1;
+      for (int ch = 0; ch < nch; ++ch)
+        timed(2, [&] { fops.q_syn[o](c, o, band + ch, 0, nch, R[ch].p, djobs.p + jb, 1); });
       return;
     }
     const NSItem &it = T.items[i];
@@ -420,13 +604,12 @@ struct surfhip_ns_plan {
   }
   void finish(float *x) {
     static const NSItem none{};
-    if (fast) {
-      close_open();
-      const FastCtx c = ctx();
-      for (int ch = 0; ch < nch; ++ch) {
-        ops.a_inv(c, acc[ch].p, Vfin.p);
-        ops.b_inv(c, Vfin.p, 1, x + ch, 0, (size_t)nch, 1.0f / (float)N, fullsup);
-      }
+    if (fused) {
+      // Slabs that received no subband hold no partial sum: clear them.
+      for (int u = 0; u < nslabs; ++u)
+        if (!touched[u])
+          for (int ch = 0; ch < nch; ++ch) timed(4, [&] { zero(R[ch].p + (int64_t)u * H, H); });
+      for (int ch = 0; ch < nch; ++ch) finish_channel(ch, x);
       return;
     }
     for (int ch = 0; ch < nch; ++ch) {
@@ -499,8 +682,8 @@ int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, in
     P->f1 = make_fft_plan(n1, pool);
     P->f2 = make_fft_plan(n2, pool);
     P->f3 = make_fft_plan(n3, pool);
-    P->fast = std::getenv("SURFHIP_NS_GENERIC") == nullptr && fast_ops_for(n1, n2, n3, P->ops);
-    if (!P->fast) {
+    P->fused = std::getenv("SURFHIP_NS_GENERIC") == nullptr && fused_ops_for(n1, n2, n3, P->fops);
+    if (!P->fused) {
       P->g_rows = std::max(2, std::min(16, pick_group(n1)));
       P->g_axis = std::min(pick_group(std::max(n2, n3)), 16);
       check_lds(P->lds_rows_r2c(), "the axis-1 transform");
@@ -509,38 +692,23 @@ int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, in
       check_lds(P->lds_axis(n3), "the axis-3 transform");
     }
     P->tw.upload(pool);
-    P->hg1.upload(P->T.hg1);
-    P->hg1c.upload(P->T.hg1c);
-    P->hg2.upload(P->T.hg2);
-    P->hg2c.upload(P->T.hg2c);
-    P->a.upload(P->T.a);
-    P->l.upload(P->T.l);
-    P->c.upload(P->T.c);
+    P->gana.upload(P->T.gana);
+    if (!P->T.gsyn.empty()) P->gsyn.upload(P->T.gsyn);
     if (P->T.planes.empty()) P->T.planes.push_back(0.0f);
     P->planes.upload(P->T.planes);
     P->ditems.upload(P->T.items);
-    P->dview = NSMaskView{P->hg1.p, P->hg1c.p, P->hg2.p, P->hg2c.p, P->a.p, P->l.p, P->c.p, P->planes.p,
-                          P->h1,    n2,         n3,        P->T.J,    P->T.tight, 0, 0};
-    for (int ch = 0; ch < P->nch; ++ch) {
-      P->X[ch].alloc(P->H);
-      P->acc[ch].alloc(P->H);
-    }
-    if (P->fast) {
-      P->compute_supports();
-      if (std::getenv("SURFHIP_NS_NO_PRUNE")) {
-        for (auto &g : P->gsup) g = P->fullsup;
-        P->lowsup = P->fullsup;
-      }
-      for (int c : P->T.group_count) P->maxw = std::max(P->maxw, c);
-      P->VT.alloc((size_t)P->maxw * P->H);
-      P->Vfin.alloc(P->H);
-      for (int ch = 0; ch < P->nch; ++ch) P->accg[ch].alloc(P->H);
+    P->dview = NSMaskView{P->gana.p, P->T.gsyn.empty() ? P->gana.p : P->gsyn.p, P->planes.p, P->h1, n2, n3,
+                          P->T.ngroups, P->T.tight, 0, 0};
+    for (int ch = 0; ch < P->nch; ++ch) P->X[ch].alloc(P->H);
+    if (P->fused) {
+      P->build_fused();
     } else {
+      for (int ch = 0; ch < P->nch; ++ch) P->acc[ch].alloc(P->H);
       P->work.alloc(P->H);
     }
-    // The host copies of the big tables are no longer needed.
-    std::vector<float>().swap(P->T.hg1);
-    std::vector<float>().swap(P->T.hg1c);
+    // The host copies of the group tables are no longer needed.
+    std::vector<float>().swap(P->T.gana);
+    std::vector<float>().swap(P->T.gsyn);
     SH_HIP(hipDeviceSynchronize());
     *out = P.release();
   });
@@ -602,7 +770,7 @@ int surfhip_ns_stream_subband(surfhip_ns_plan *plan, int32_t i, void *band) {
     if (!plan || !band) fail(SURFHIP_ERR_ARG, "null argument");
     if (!plan->streaming_fwd) fail(SURFHIP_ERR_ARG, "surfhip_ns_stream_begin was not called");
     if (i < 1 || i > plan->T.S) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", i, plan->T.S));
-    plan->subbands(i - 1, 1, (float *)band);
+    plan->stream_subband(i - 1, (float *)band);
   });
 }
 
@@ -648,7 +816,7 @@ int surfhip_ns_timing(surfhip_ns_plan *plan, int64_t *launches, double *ms, char
       if (launches) launches[c] = plan->tcount[c];
       if (ms) ms[c] = plan->tms[c];
     }
-    const char *s = plan->fast ? "pencil_analysis,plane_analysis,plane_synthesis,pencil_synthesis,other"
+    const char *s = plan->fused ? "pencil_analysis,plane_inverse,plane_forward,pencil_synthesis,other"
                                : "unclassified,unclassified,unclassified,unclassified,unclassified";
     if (names && n) {
       size_t m = std::min(n - 1, std::strlen(s));

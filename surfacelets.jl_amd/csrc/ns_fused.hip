@@ -1,0 +1,880 @@
+// Factored NSSurfacelet kernels for power-of-two grids with n1, n2, n3 in
+// {32, 64, 128} (the factorisation is described in ns_fused.hpp).
+//
+// Pencil passes (one 1-D transform along the group's pencil axis per block
+// of TP pencils, a "tile"): the spectra X / acc are pencil-major
+// [k1][k2][k3]; the slabs are plane-major, so a tile reads and writes rows
+// of TP consecutive elements.  Plane passes (two 1-D transforms per block of
+// one plane): a slab plane holds rows k1 <= n1/2 of the tile axis; the real
+// transform along axis 1 runs on column pairs (two real columns per complex
+// FFT), and rows k1 = 0 and k1 = n1/2, whose transforms enter the real
+// transform through their real parts only, share one complex FFT, so a block
+// needs n1/2 row slots.  Every FFT is the four-step register FFT of
+// fft_reg.hpp with one padded LDS exchange.
+//
+// Addresses are formed as block-uniform base pointers plus 32-bit offsets.
+// Every kernel asks for at least 4 waves per SIMD (at most 128 VGPRs): two
+// 8-wave plane blocks or four 4-wave pencil blocks per CU, which is what
+// their LDS allows.
+//
+// Bitwise reproducibility: the streaming entry points run these same
+// kernels with smaller job or unit lists; the arithmetic of a subband never
+// depends on which other subbands share the launch, and every sum has a
+// fixed order with explicit FMAs.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fft_reg.hpp"
+#include "hip_util.hpp"
+#include "ns_fused.hpp"
+#include "ns_tables.hpp"
+
+namespace surfhip {
+
+// Pencils per tile and threads per pencil block (R2 threads per pencil).
+template <int N> constexpr int tp_of() { return 16; }
+template <int N> constexpr int pencil_threads() { return tp_of<N>() * FourStep<N>::R2; }
+// LDS slots of one pencil: the padded exchange needs N + N/R1, and an odd
+// stride keeps accesses of neighbouring pencils in different banks.
+template <int N> constexpr int pencil_stride() { return N + N / FourStep<N>::R1 + 1; }
+
+// Plane blocks: n1/2 row slots of the tile axis (slot 0 carries rows 0 and
+// n1/2), the pair-contiguous columns of the axis-1 transform, the real output.
+template <int N1, int NC> constexpr int plane_slots() {
+  constexpr int a = (N1 / 2) * pencil_stride<NC>(), b = (NC / 2) * pencil_stride<N1>(), c = N1 * NC / 2;
+  constexpr int m = a > b ? a : b;
+  return m > c ? m : c;
+}
+template <int N1, int NC> constexpr int plane_threads() {
+  constexpr int a = (N1 / 2) * FourStep<NC>::R2, b = (NC / 2) * FourStep<N1>::R2;
+  constexpr int m = a > b ? a : b;
+  return (m + 63) / 64 * 64;
+}
+
+// Exchange slot of element e: one pad slot per R1, so that the R1
+// consecutive slots each thread reads in phase 2 start in different banks.
+template <int R1> __device__ __forceinline__ int xpos(int e) { return e + e / R1; }
+
+// A copy of v the compiler cannot see through: index math derived from it
+// inside a loop is recomputed per iteration instead of being hoisted and kept
+// live (in registers) across the loop.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// base[byte offset]: a block-uniform base plus a 32-bit byte offset, the
+// form the scalar-base global loads and stores take.
+template <class T> __device__ __forceinline__ const T &at_b(const T *base, unsigned bytes) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + bytes);
+}
+template <class T> __device__ __forceinline__ T &at_b(T *base, unsigned bytes) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + bytes);
+}
+
+__device__ __forceinline__ float2 fma2(float2 a, float s, float2 c) {
+  return make_float2(__fmaf_rn(a.x, s, c.x), __fmaf_rn(a.y, s, c.y));
+}
+
+// Mirror X[(N - k) mod N] of element (u, q) of a pencil thread after
+// RegFFT::phase2 (v[u R2 + q] = X[(j ROWS + u) + R1 q]).  The R2 threads of a
+// pencil are consecutive lanes, and the mirror of element (u, q) of thread j
+// is element (u, R2 - 1 - q) of thread (R1 - j ROWS - u) / ROWS (element 0 of
+// thread 0 mirrors its own element (0, -q mod R2)), so one shuffle per
+// component fetches it.  Every lane of the pencil must call it.
+template <int N>
+__device__ __forceinline__ float2 fft_mirror(const float2 (&v)[FourStep<N>::R1], int j, int u, int q) {
+  constexpr int R1 = FourStep<N>::R1, R2 = FourStep<N>::R2, ROWS = R1 / R2;
+  const int s = j * ROWS + u;
+  const int src = s == 0 ? 0 : (R1 - s) / ROWS;
+  const float2 x = v[u * R2 + (R2 - 1 - q)];
+  float2 y = make_float2(__shfl(x.x, src, R2), __shfl(x.y, src, R2));
+  if (u == 0 && j == 0) y = v[(R2 - q) % R2];
+  return y;
+}
+
+// In-place FFT of one pencil at pb (natural order in slots 0..N-1) by its R2
+// threads j; every thread of the block calls it (it holds barriers).
+template <int N, int DIR>
+__device__ __forceinline__ void pencil_fft(float2 *pb, const float2 *tw, int j, bool active) {
+  using F = RegFFT<N, DIR>;
+  constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS;
+  float2 v[R1];
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = pb[j + R2 * r];
+    F::phase1(v, j, tw);
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
+    F::phase2(v);
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int u = 0; u < ROWS; ++u)
+#pragma unroll
+      for (int q = 0; q < R2; ++q) pb[(j * ROWS + u) + R1 * q] = v[u * R2 + q];
+  }
+  __syncthreads();
+}
+
+// One wedge plane as seen from row k1: value at (k2, k3) = p[k2 s2 + k3 s3],
+// or 1 everywhere (unit) for an absent plane, the lowpass or a unit wedge.
+struct PlaneRef {
+  const float *p;
+  int s2, s3;
+  bool unit;
+  __device__ __forceinline__ float at(int k2, int k3) const {
+    const float v = p[(unsigned)(k2 * s2 + k3 * s3)];
+    return unit ? 1.0f : v;
+  }
+};
+__device__ __forceinline__ PlaneRef plane_ref(const NSMaskView &v, int32_t off, int32_t pair, int k1, bool unit) {
+  PlaneRef r{v.planes, 0, 0, true};
+  if (off < 0 || unit) return r;
+  r.unit = false;
+  if (pair == 0) {
+    r.p = v.planes + off + (size_t)k1 * v.n2;
+    r.s2 = 1;
+  } else if (pair == 1) {
+    r.p = v.planes + off + (size_t)k1 * v.n3;
+    r.s3 = 1;
+  } else {
+    r.p = v.planes + off;
+    r.s2 = v.n3;
+    r.s3 = 1;
+  }
+  return r;
+}
+__device__ __forceinline__ PlaneRef wedge_b_ref(const NSMaskView &v, const NSItem &it, int k1, bool analysis) {
+  return plane_ref(v, it.pb_off, it.pb_pair, k1, it.lowpass || ns_unit_wedge(v, analysis));
+}
+
+// ------------------------------------------------------------ pencil passes
+//
+// A pencil block has two halves of TP R2 threads; each half transforms one
+// slab's tile at a time (consecutive b-indices q, q + 1), and the tile's
+// elements are shared out over all threads of the block (TileElem).
+
+template <int NP> constexpr int half_threads() { return tp_of<NP>() * FourStep<NP>::R2; }
+template <int NP> constexpr int pblock_threads() { return 2 * half_threads<NP>(); }
+
+// Element m of a thread's share of a tile (TP pencils of length NP), chosen so
+// that a warp touches contiguous memory of the pencil-major spectrum: along
+// the pencil for AX3 (pencils are contiguous), across the tile for AX2.
+// `off` is the element's offset inside the k1 slab of an [k1][k2][k3] array.
+template <int NP, int OR> struct TileElem {
+  int pp, kp, k2, k3;
+  unsigned off;
+  __device__ __forceinline__ TileElem(int tid, int m, int c0, int n3) {
+    constexpr int TP = tp_of<NP>(), NTH = pblock_threads<NP>();
+    const int e = tid + NTH * m;
+    if (OR == AX3) {
+      pp = e / NP;
+      kp = e % NP;
+      k2 = c0 + pp;
+      k3 = kp;
+    } else {
+      pp = e % TP;
+      kp = e / TP;
+      k2 = kp;
+      k3 = c0 + pp;
+    }
+    off = (unsigned)(k2 * n3 + k3);
+  }
+};
+
+// Inverse (DIR = +1) or forward FFT of the TP pencils in buf (natural order)
+// by one half-block: thread (pq, j); the result stays in registers in phase-2
+// order, v[uu R2 + qq] = element (j ROWS + uu) + R1 qq of pencil pq.
+template <int NP, int DIR>
+__device__ __forceinline__ void tile_fft(float2 *pb, const float2 *tw, int j, bool active,
+                                         float2 (&v)[FourStep<NP>::R1]) {
+  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2;
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = pb[j + R2 * r];
+    RegFFT<NP, DIR>::phase1(v, j, tw);
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
+    RegFFT<NP, DIR>::phase2(v);
+  }
+}
+
+// Analysis pencil pass.  Unit {g, k1, c0}: Y = X G_g over the tile, then for
+// each live b-index q: slab_q[i][k1][c0 + pp] = IFFT_pencil(Y w_b,q)[i].
+template <int NP, int OR>
+__global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *__restrict__ X,
+                                                                  float2 *__restrict__ slabs, int64_t H, int n2,
+                                                                  int n3, NSMaskView mv,
+                                                                  const NSItem *__restrict__ items,
+                                                                  const GroupDev *__restrict__ groups,
+                                                                  const uint32_t *__restrict__ live,
+                                                                  const int4 *__restrict__ units,
+                                                                  const float2 *__restrict__ twg) {
+  constexpr int TP = tp_of<NP>(), NTH = pblock_threads<NP>(), HT = half_threads<NP>(), PS = pencil_stride<NP>();
+  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2, ROWS = R1 / R2, E = TP * NP / NTH;
+  __shared__ float2 buf[2][TP * PS];
+  __shared__ float2 tw[NP];
+  const int tid = threadIdx.x;
+  const int4 un = units[blockIdx.x];
+  const int g = un.x, k1 = un.y, c0 = un.z, tile = c0 / TP;
+  const GroupDev gd = groups[g];
+  const int h1 = mv.h1, nt = OR == AX3 ? n2 : n3;
+  for (int m = tid; m < NP; m += NTH) tw[m] = twg[m];
+  const size_t kb = (size_t)k1 * n2 * n3;
+  const float2 *Xk = X + kb;
+  const float *Gk = ns_group_table(mv, g, true) + kb;
+  float2 y[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const TileElem<NP, OR> t(tid, m, c0, n3);
+    y[m] = cscale(Xk[t.off], Gk[t.off]);
+  }
+  const int half = tid / HT, lt = tid % HT, pq = lt % TP, j = lt / TP;
+  const int rs = h1 * nt;
+  for (int q0 = 0; q0 < gd.nB; q0 += 2) {
+    bool lv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = q0 + h;
+      lv[h] = q < gd.nB && ((live[(size_t)(gd.slab0 + q) * h1 + k1] >> tile) & 1u);
+      if (!lv[h]) continue;
+      const PlaneRef wb = wedge_b_ref(mv, items[gd.first + q], k1, true);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const TileElem<NP, OR> t(tid, m, c0, n3);
+        buf[h][t.pp * PS + t.kp] = cscale(y[m], wb.at(t.k2, t.k3));
+      }
+    }
+    __syncthreads();
+    const bool act = lv[half];
+    float2 v[R1];
+    tile_fft<NP, 1>(buf[half] + pq * PS, tw, j, act, v);
+    if (act) {
+      float2 *dst = slabs + (size_t)(gd.slab0 + q0 + half) * H + (size_t)k1 * nt + c0;
+#pragma unroll
+      for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+        for (int qq = 0; qq < R2; ++qq)
+          dst[(unsigned)(((j * ROWS + uu) + R1 * qq) * rs + pq)] = v[uu * R2 + qq];
+    }
+    __syncthreads();
+  }
+}
+
+// Synthesis pencil pass over one tile (k1, c0) of orientation OR:
+//   a = acc (AX3 with has_acc) or 0;
+//   for g in glist, for each live q: a = fma(FFT_pencil(slab_q), G_g w_b,q, a)
+// in that order.  AX2 writes a into acc; AX3 writes IFFT_3(a) into V
+// ([i3][k1][k2]).
+template <int NP, int OR>
+__global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_syn(const float2 *__restrict__ slabs, int64_t H,
+                                                                  int n2, int n3, NSMaskView mv,
+                                                                  const NSItem *__restrict__ items,
+                                                                  const GroupDev *__restrict__ groups,
+                                                                  const uint32_t *__restrict__ live,
+                                                                  const int32_t *__restrict__ glist, int ng,
+                                                                  float2 *__restrict__ acc, int has_acc,
+                                                                  float2 *__restrict__ V,
+                                                                  const float2 *__restrict__ twg) {
+  constexpr int TP = tp_of<NP>(), NTH = pblock_threads<NP>(), HT = half_threads<NP>(), PS = pencil_stride<NP>();
+  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2, ROWS = R1 / R2, E = TP * NP / NTH;
+  __shared__ float2 buf[2][TP * PS];
+  __shared__ float2 tw[NP];
+  const int tid = threadIdx.x;
+  const int h1 = mv.h1, nt = OR == AX3 ? n2 : n3, ntiles = nt / TP;
+  const int k1 = blockIdx.x / ntiles, tile = blockIdx.x % ntiles, c0 = tile * TP;
+  for (int m = tid; m < NP; m += NTH) tw[m] = twg[m];
+  const size_t kb = (size_t)k1 * n2 * n3;
+  float2 a[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const TileElem<NP, OR> t(tid, m, c0, n3);
+    a[m] = (OR == AX3 && has_acc) ? acc[kb + t.off] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  const int half = tid / HT, lt = tid % HT, pq = lt % TP, j = lt / TP;
+  const int rs = h1 * nt;
+  for (int gi = 0; gi < ng; ++gi) {
+    const int g = glist[gi];
+    const GroupDev gd = groups[g];
+    const float *Gk = ns_group_table(mv, g, false) + kb;
+    for (int q0 = 0; q0 < gd.nB; q0 += 2) {
+      bool lv[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        lv[h] = q0 + h < gd.nB && ((live[(size_t)(gd.slab0 + q0 + h) * h1 + k1] >> tile) & 1u);
+      if (!lv[0] && !lv[1]) continue;
+      const bool act = lv[half];
+      float2 v[R1];
+      if (act) {
+        const float2 *src = slabs + (size_t)(gd.slab0 + q0 + half) * H + (size_t)k1 * nt + c0;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) v[r] = src[(unsigned)((j + R2 * r) * rs + pq)];
+      }
+      float2 *pb = buf[half] + pq * PS;
+      if (act) {
+        RegFFT<NP, -1>::phase1(v, j, tw);
+#pragma unroll
+        for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
+      }
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
+        RegFFT<NP, -1>::phase2(v);
+      }
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+          for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (!lv[h]) continue;
+        const PlaneRef wb = wedge_b_ref(mv, items[gd.first + q0 + h], k1, false);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const TileElem<NP, OR> t(tid, m, c0, n3);
+          a[m] = fma2(buf[h][t.pp * PS + t.kp], Gk[t.off] * wb.at(t.k2, t.k3), a[m]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (OR == AX2) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const TileElem<NP, OR> t(tid, m, c0, n3);
+      acc[kb + t.off] = a[m];
+    }
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const TileElem<NP, OR> t(tid, m, c0, n3);
+    buf[0][t.pp * PS + t.kp] = a[m];
+  }
+  __syncthreads();
+  float2 v[R1];
+  tile_fft<NP, 1>(buf[0] + pq * PS, tw, j, half == 0, v);
+  if (half == 0) {
+    float2 *dst = V + (size_t)k1 * n2 + c0;
+    const int vs = h1 * n2;
+#pragma unroll
+    for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+      for (int qq = 0; qq < R2; ++qq) dst[(unsigned)(((j * ROWS + uu) + R1 * qq) * vs + pq)] = v[uu * R2 + qq];
+  }
+}
+
+// X[k1][k2][:] = FFT_3(T[:][k1][k2]) (T plane-major [i3][k1][k2]).
+template <int NP>
+__global__ __launch_bounds__(pencil_threads<NP>()) void kP_fwd(const float2 *__restrict__ T, float2 *__restrict__ X,
+                                                               int h1, int n2, const float2 *__restrict__ twg) {
+  constexpr int TP = tp_of<NP>(), NTH = pencil_threads<NP>(), PS = pencil_stride<NP>();
+  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2, ROWS = R1 / R2, E = TP * NP / NTH;
+  __shared__ float2 buf[TP * PS];
+  __shared__ float2 tw[NP];
+  const int tid = threadIdx.x, ntiles = n2 / TP;
+  const int k1 = blockIdx.x / ntiles, c0 = (blockIdx.x % ntiles) * TP;
+  for (int m = tid; m < NP; m += NTH) tw[m] = twg[m];
+  __syncthreads();
+  const int pq = tid % TP, j = tid / TP;
+  const float2 *src = T + (size_t)k1 * n2 + c0;
+  const int rs = h1 * n2;
+  float2 v[R1];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) v[r] = src[(unsigned)((j + R2 * r) * rs + pq)];
+  RegFFT<NP, -1>::phase1(v, j, tw);
+  float2 *pb = buf + pq * PS;
+#pragma unroll
+  for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
+  RegFFT<NP, -1>::phase2(v);
+  __syncthreads();
+#pragma unroll
+  for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+    for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
+  __syncthreads();
+  float2 *dst = X + ((size_t)k1 * n2 + c0) * NP;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int e = tid + NTH * m, pp = e / NP, kp = e % NP;
+    dst[e] = buf[pp * PS + kp];
+  }
+}
+
+// ------------------------------------------------------------- plane passes
+//
+// Plane ip of a slab of orientation AX3 is i3 = ip with columns k2 (NC = n2);
+// of AX2, i2 = ip with columns k3 (NC = n3).  In the volume, column ic of the
+// plane starts at pbase + ic cstr: AX3 pbase = ip n1 n2, cstr = n1; AX2
+// pbase = ip n1, cstr = n1 n2.  Plane a of a group is [k1][kc] in the pool
+// for both orientations (pair (1,2) for AX3, (1,3) for AX2).
+
+// Analysis plane pass, block (ip, job) for the single a-index p = p0:
+// out plane = scale * c2r_1(IFFT_c(U w_a,p)), U the slab plane.  Row k1 of
+// the plane is transformed by the R2 threads of slot k1; the results go to
+// LDS column-major (column stride CS) for the transform along k1, whose
+// outputs are stored straight from registers.
+template <int NC> constexpr int col_stride() { return NC >= 64 ? 68 : 36; }
+template <int N1, int NC> constexpr int ana_slots() {
+  constexpr int a = (N1 / 2 + 1) * pencil_stride<NC>(), b = NC * col_stride<N1>(), c = (NC / 2) * pencil_stride<N1>();
+  constexpr int m = a > b ? a : b;
+  return m > c ? m : c;
+}
+template <int N1, int NC> constexpr int ana_threads() {
+  constexpr int a = (N1 / 2 + 1) * FourStep<NC>::R2, b = (NC / 2) * FourStep<N1>::R2;
+  constexpr int m = a > b ? a : b;
+  return (m + 63) / 64 * 64;
+}
+
+template <int N1, int NC, int OR>
+__global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
+    const float2 *__restrict__ slabs, int64_t H, int n1n2, NSMaskView mv, const uint32_t *__restrict__ live,
+    const PlaneJob *__restrict__ jobs, float *__restrict__ out, int64_t substride, int os, float scale,
+    const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg, int tpw) {
+  constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = pencil_stride<NC>(), PP = pencil_stride<N1>();
+  constexpr int CS = col_stride<N1>();
+  constexpr int NTH = ana_threads<N1, NC>();
+  constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
+  __shared__ __align__(16) float2 L[ana_slots<N1, NC>()];
+  __shared__ float2 tw1[N1], twc[NC];
+  __shared__ uint32_t rl[H1];
+  const int tid = threadIdx.x, ip = blockIdx.x;
+  const PlaneJob jb = jobs[blockIdx.y];
+  const bool masked = jb.slab >= 0 && jb.group >= 0;
+  const int slot = tid / RC2, j = tid % RC2;
+  const bool onslot = slot < H1;
+  // Issue the slab and mask loads first; dead tiles are selected away below.
+  const int p = jb.p0;
+  const bool wunit = !masked || jb.pa0 < 0 || ns_unit_wedge(mv, true);
+  const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
+  const float2 *Ub = slabs + (size_t)(jb.slab < 0 ? 0 : jb.slab) * H + (size_t)ip * H1 * NC;
+  float2 u[RC1];
+  float wa[RC1];
+  const int row = onslot ? slot : 0;
+#pragma unroll
+  for (int r = 0; r < RC1; ++r) {
+    const unsigned o = (unsigned)(row * NC + j + RC2 * r);
+    u[r] = Ub[o];
+    wa[r] = wab[wunit ? 0u : o];
+  }
+  for (int m = tid; m < N1; m += NTH) tw1[m] = tw1g[m];
+  for (int m = tid; m < NC; m += NTH) twc[m] = twcg[m];
+  for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
+  __syncthreads();
+  const uint32_t lrow = onslot ? rl[slot] : 0u;
+  const bool act = lrow != 0u;
+  float2 *pb = L + row * PS;
+  {
+    float2 v[RC1];
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < RC1; ++r) {
+        const bool lv = ((lrow >> ((j + RC2 * r) / tpw)) & 1u) != 0;
+        v[r] = lv ? cscale(u[r], wunit ? 1.0f : wa[r]) : make_float2(0.0f, 0.0f);
+      }
+      RegFFT<NC, 1>::phase1(v, j, twc);
+#pragma unroll
+      for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + j)] = v[s];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(j * RC1 + e)];
+      RegFFT<NC, 1>::phase2(v);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int uu = 0; uu < RCR; ++uu)
+#pragma unroll
+        for (int qq = 0; qq < RC2; ++qq) L[((j * RCR + uu) + RC1 * qq) * CS + slot] = v[uu * RC2 + qq];
+    }
+    __syncthreads();
+  }
+  // c2r along k1 on column pairs (2 pr, 2 pr + 1): z = a + i b through the
+  // Hermitian extension (imaginary parts of rows 0 and n1/2 ignored); rows
+  // that hold no data are zero.
+  using F = RegFFT<N1, 1>;
+  constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS;
+  const int pr = tid / R2, jj = tid % R2;
+  const bool on = pr < NC / 2;
+  float2 w[R1];
+  if (on) {
+    const float2 *ca = L + (2 * pr) * CS, *cb = ca + CS;
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      const int t = jj + R2 * r;
+      const int tt = t <= HS ? t : N1 - t;
+      float2 A = make_float2(0.0f, 0.0f), B = A;
+      if (rl[tt]) {
+        A = ca[tt];
+        B = cb[tt];
+      }
+      if (t == 0 || t == HS) {
+        A.y = 0.0f;
+        B.y = 0.0f;
+      } else if (t > HS) {
+        A = conjf2(A);
+        B = conjf2(B);
+      }
+      w[r] = make_float2(A.x - B.y, A.y + B.x);
+    }
+    F::phase1(w, jj, tw1);
+  }
+  __syncthreads();
+  float2 *qb = L + (on ? pr : 0) * PP;
+  if (on) {
+#pragma unroll
+    for (int s = 0; s < R1; ++s) qb[xpos<R1>(s * R2 + jj)] = w[s];
+  }
+  __syncthreads();
+  if (!on) return;
+#pragma unroll
+  for (int e = 0; e < R1; ++e) w[e] = qb[xpos<R1>(jj * R1 + e)];
+  F::phase2(w);
+  // Thread (pr, jj) holds columns 2 pr (.x) and 2 pr + 1 (.y) at
+  // i1 = (jj ROWS + u) + R1 q.
+  const int pbase = OR == AX3 ? ip * n1n2 : ip * N1;
+  const int cstr = OR == AX3 ? N1 : n1n2;
+  float *dst = out + (int64_t)jb.out0 * substride + (int64_t)pbase * os;
+  if (ROWS == 2 && os == 1 && (reinterpret_cast<uintptr_t>(dst) & 7) == 0 && (cstr & 1) == 0) {
+    float *d0 = dst + (unsigned)((2 * pr) * cstr), *d1 = d0 + cstr;
+#pragma unroll
+    for (int q = 0; q < R2; ++q) {
+      const int i1 = jj * 2 + R1 * q;
+      const float2 z0 = w[q], z1 = w[R2 + q];
+      *reinterpret_cast<float2 *>(d0 + i1) = make_float2(z0.x * scale, z1.x * scale);
+      *reinterpret_cast<float2 *>(d1 + i1) = make_float2(z0.y * scale, z1.y * scale);
+    }
+  } else {
+#pragma unroll
+    for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+      for (int q = 0; q < R2; ++q) {
+        const int i1 = (jj * ROWS + uu) + R1 * q;
+        const float2 z = w[uu * R2 + q];
+        dst[(int64_t)((2 * pr) * cstr + i1) * os] = z.x * scale;
+        dst[(int64_t)((2 * pr + 1) * cstr + i1) * os] = z.y * scale;
+      }
+  }
+}
+
+// Element e = tid + NTH m of a real plane as column pairs: pair e / N1, row
+// i1 = e % N1 of columns 2 pair (a) and 2 pair + 1 (b); a warp reads
+// consecutive i1 of both columns.  sp: the plane's first sample.  NTH is a
+// multiple of N1, so element m of a thread is its element 0 moved by the
+// block-uniform m (NTH / N1) column pairs: one 32-bit offset per thread.
+template <int N1, int NC, int NTH, int NLD>
+__device__ __forceinline__ void plane_pairs_load(const float *__restrict__ sp, int tid, int cstr, int is,
+                                                 float (&a)[NLD], float (&b)[NLD]) {
+  static_assert(NTH % N1 == 0 && NLD * NTH == N1 * NC / 2, "plane load shape");
+  const unsigned o = (unsigned)(((tid / N1) * 2 * cstr + tid % N1) * is) * 4u;
+  const unsigned ob = o + (unsigned)(cstr * is) * 4u;
+  const int64_t step = (int64_t)(NTH / N1) * 2 * cstr * is;
+#pragma unroll
+  for (int m = 0; m < NLD; ++m) {
+    const float *spm = sp + m * step;
+    a[m] = at_b(spm, o);
+    b[m] = at_b(spm, ob);
+  }
+}
+
+// Synthesis plane pass, block (ip, job): R = (init ? 0 : slab plane), then for
+// p in [p0, p1): R = fma(FFT_c(r2c_1(in_p plane)), w_a,p, R); R -> slab plane
+// (rows k1 <= n1/2, live tiles).  The transform along axis 1 runs on column
+// pairs and is unpacked in registers (the mirror Z[-k] comes from a lane of
+// the same pencil).  Rows 0 and n1/2 of r2c_1 are real, so slot 0 transforms
+// row 0 + i row n1/2 and splits the result the same way; the row n1/2 sum
+// lives in LDS (R64).
+template <int N1, int NC, int OR>
+__global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
+    const float *__restrict__ in, int64_t substride, int is, int n1n2, NSMaskView mv,
+    const uint32_t *__restrict__ live, const PlaneJob *__restrict__ jobs, float2 *__restrict__ slabs, int64_t H,
+    const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg, int tpw) {
+  constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = pencil_stride<NC>(), PP = pencil_stride<N1>();
+  constexpr int NTH = plane_threads<N1, NC>();
+  constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
+  constexpr int C1 = FourStep<N1>::R1, C2 = FourStep<N1>::R2, CR = C1 / C2;
+  constexpr int NLD = N1 * NC / 2 / NTH;  // column-pair elements per thread
+  __shared__ __align__(16) float2 L[plane_slots<N1, NC>()];
+  __shared__ float2 tw1[N1], twc[NC], R64[NC];
+  __shared__ uint32_t rl[H1];
+  const int tid = threadIdx.x, ip = blockIdx.x;
+  const PlaneJob jb = jobs[blockIdx.y];
+  const bool masked = jb.group >= 0;
+  for (int m = tid; m < N1; m += NTH) tw1[m] = tw1g[m];
+  for (int m = tid; m < NC; m += NTH) twc[m] = twcg[m];
+  for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
+  __syncthreads();
+  const int slot = tid / RC2, j = tid % RC2;
+  const bool onslot = slot < HS;
+  const uint32_t lrow = onslot ? (slot == 0 ? (rl[0] | rl[HS]) : rl[slot]) : 0u;
+  const bool act = lrow != 0u;
+  float2 *Sb = slabs + (size_t)jb.slab * H + (size_t)ip * H1 * NC;
+  // Running sums in the row FFT's output order: element (uu, qq) of thread
+  // (slot, j) is column kc = (j RCR + uu) + RC1 qq (row HS in R64).
+  float2 R[RC1];
+#pragma unroll
+  for (int uu = 0; uu < RCR; ++uu)
+#pragma unroll
+    for (int qq = 0; qq < RC2; ++qq) {
+      const int kc = (j * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
+      R[m] = make_float2(0.0f, 0.0f);
+      if (act && !jb.init && ((rl[slot] >> (kc / tpw)) & 1u)) R[m] = Sb[(unsigned)(slot * NC + kc)];
+      if (act && slot == 0)
+        R64[kc] = (!jb.init && ((rl[HS] >> (kc / tpw)) & 1u)) ? Sb[(unsigned)(HS * NC + kc)] : make_float2(0.0f, 0.0f);
+    }
+  const int pbase = OR == AX3 ? ip * n1n2 : ip * N1;
+  const int cstr = OR == AX3 ? N1 : n1n2;
+  const bool wunit = !masked || jb.pa0 < 0 || ns_unit_wedge(mv, false);
+  for (int p = jb.p0; p < jb.p1; ++p) {
+    const int t = opaque(tid);
+    {
+      // Columns (2 pr, 2 pr + 1) of the real plane -> pair pr, pair-contiguous.
+      float a[NLD], b[NLD];
+      plane_pairs_load<N1, NC, NTH, NLD>(
+          in + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * is, t, cstr, is, a, b);
+      float2 *lp = L + (t / N1) * PP + t % N1;
+#pragma unroll
+      for (int m = 0; m < NLD; ++m) lp[m * (NTH / N1) * PP] = make_float2(a[m], b[m]);
+    }
+    __syncthreads();
+    {
+      // FFT along k1 of z = col 2 pr + i col 2 pr + 1, then rows k <= HS:
+      // col 2 pr gets A = (Z[k] + conj Z[-k]) / 2, col 2 pr + 1 gets
+      // B = (Z[k] - conj Z[-k]) / 2i; rows 0 and HS (real) go to slot 0 as
+      // row 0 + i row HS.
+      const int pr = t / C2, jj = t % C2;
+      const bool on = pr < NC / 2;
+      float2 *qb = L + (on ? pr : 0) * PP;
+      float2 v[C1];
+      if (on) {
+#pragma unroll
+        for (int r = 0; r < C1; ++r) v[r] = qb[jj + C2 * r];
+        RegFFT<N1, -1>::phase1(v, jj, tw1);
+      }
+      __syncthreads();
+      if (on) {
+#pragma unroll
+        for (int s = 0; s < C1; ++s) qb[xpos<C1>(s * C2 + jj)] = v[s];
+      }
+      __syncthreads();
+      if (on) {
+#pragma unroll
+        for (int e = 0; e < C1; ++e) v[e] = qb[xpos<C1>(jj * C1 + e)];
+        RegFFT<N1, -1>::phase2(v);
+      }
+      __syncthreads();
+      {
+        float2 z0 = make_float2(0.0f, 0.0f), zh = z0;
+#pragma unroll
+        for (int u = 0; u < CR; ++u)
+#pragma unroll
+          for (int q = 0; q < C2; ++q) {
+            const int k = (jj * CR + u) + C1 * q, m = u * C2 + q;
+            // Rows k <= HS need the mirror; with C1 q > HS no lane does.
+            if (C1 * q > HS) continue;
+            const float2 Zm = fft_mirror<N1>(v, jj, u, q);
+            if (!on || k > HS) continue;
+            if (k == 0) z0 = v[m];
+            if (k == HS) zh = v[m];
+            if (k == 0 || k == HS || !rl[k]) continue;
+            const float2 Z = v[m], Zc = conjf2(Zm);
+            const float2 d = csub(Z, Zc);
+            L[k * PS + 2 * pr] = cscale(cadd(Z, Zc), 0.5f);
+            L[k * PS + 2 * pr + 1] = make_float2(0.5f * d.y, -0.5f * d.x);
+          }
+        // Thread jj = 0 holds k = 0 and k = HS (HS is a multiple of C1).
+        if (on && jj == 0) {
+          L[2 * pr] = make_float2(z0.x, zh.x);
+          L[2 * pr + 1] = make_float2(z0.y, zh.y);
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const int sl = t / RC2, jr = t % RC2;
+      float2 *pb = L + (onslot ? sl : 0) * PS;
+      float2 v[RC1];
+      if (act) {
+#pragma unroll
+        for (int r = 0; r < RC1; ++r) v[r] = pb[jr + RC2 * r];
+        RegFFT<NC, -1>::phase1(v, jr, twc);
+      }
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + jr)] = v[s];
+      }
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(jr * RC1 + e)];
+        RegFFT<NC, -1>::phase2(v);
+      }
+      const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
+      if (slot == 0) {
+        // Z = F(row 0) + i F(row HS), both rows real: split through Z[-kc].
+#pragma unroll
+        for (int uu = 0; uu < RCR; ++uu)
+#pragma unroll
+          for (int qq = 0; qq < RC2; ++qq) {
+            const float2 Zm = fft_mirror<NC>(v, jr, uu, qq);
+            if (act) {
+              const int kc = (jr * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
+              const float2 Z = v[m], Zc = conjf2(Zm);
+              const float2 F0 = cscale(cadd(Z, Zc), 0.5f);
+              const float2 d = csub(Z, Zc);
+              const float2 F1 = make_float2(0.5f * d.y, -0.5f * d.x);
+              const float w0 = wab[wunit ? 0u : (unsigned)kc];
+              const float w1 = wab[wunit ? 0u : (unsigned)(HS * NC + kc)];
+              R[m] = fma2(F0, wunit ? 1.0f : w0, R[m]);
+              R64[kc] = fma2(F1, wunit ? 1.0f : w1, R64[kc]);
+            }
+        }
+      } else if (act) {
+#pragma unroll
+        for (int uu = 0; uu < RCR; ++uu)
+#pragma unroll
+          for (int qq = 0; qq < RC2; ++qq) {
+            const int kc = (jr * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
+            const float wv = wab[wunit ? 0u : (unsigned)(sl * NC + kc)];
+            R[m] = fma2(v[m], wunit ? 1.0f : wv, R[m]);
+          }
+      }
+    }
+    __syncthreads();  // L is reloaded by the next subband
+  }
+  if (act) {
+#pragma unroll
+    for (int uu = 0; uu < RCR; ++uu)
+#pragma unroll
+      for (int qq = 0; qq < RC2; ++qq) {
+        const int kc = (j * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
+        if ((rl[slot] >> (kc / tpw)) & 1u) Sb[(unsigned)(slot * NC + kc)] = R[m];
+        if (slot == 0 && ((rl[HS] >> (kc / tpw)) & 1u)) Sb[(unsigned)(HS * NC + kc)] = R64[kc];
+      }
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+namespace {
+
+template <int NP, int OR> struct PencilL {
+  static void ana(const FusedCtx &c, const float2 *X, float2 *slabs, const int4 *units, int nunits, int) {
+    if (nunits <= 0) return;
+    const float2 *tw = OR == AX3 ? c.tw3 : c.tw2;
+    hipLaunchKernelGGL((kP_ana<NP, OR>), dim3(nunits), dim3(pblock_threads<NP>()), 0, c.stream, X, slabs, c.H, c.n2,
+                       c.n3, c.mv, c.items, c.groups, c.live, units, tw);
+    SH_LAUNCHED();
+  }
+  static void syn(const FusedCtx &c, int, const float2 *slabs, const int32_t *glist, int ng, float2 *acc,
+                  int has_acc, float2 *V) {
+    const int nt = OR == AX3 ? c.n2 : c.n3;
+    const float2 *tw = OR == AX3 ? c.tw3 : c.tw2;
+    hipLaunchKernelGGL((kP_syn<NP, OR>), dim3(c.h1 * (nt / tp_of<NP>())), dim3(pblock_threads<NP>()), 0, c.stream,
+                       slabs, c.H, c.n2, c.n3, c.mv, c.items, c.groups, c.live, glist, ng, acc, has_acc, V, tw);
+    SH_LAUNCHED();
+  }
+};
+
+template <int NP> void launch_fwd(const FusedCtx &c, const float2 *T, float2 *X) {
+  hipLaunchKernelGGL(kP_fwd<NP>, dim3(c.h1 * (c.n2 / tp_of<NP>())), dim3(pencil_threads<NP>()), 0, c.stream, T, X,
+                     c.h1, c.n2, c.tw3);
+  SH_LAUNCHED();
+}
+
+template <int N1, int NC, int OR> struct PlaneL {
+  static void ana(const FusedCtx &c, int, const float2 *slabs, const PlaneJob *jobs, int njobs, float *out,
+                  int64_t substride, int64_t os, float scale) {
+    if (njobs <= 0) return;
+    const int npl = OR == AX3 ? c.n3 : c.n2;
+    const float2 *twc = OR == AX3 ? c.tw2 : c.tw3;
+    hipLaunchKernelGGL((kQ_ana<N1, NC, OR>), dim3(npl, njobs), dim3(ana_threads<N1, NC>()), 0, c.stream, slabs, c.H,
+                       c.n1 * c.n2, c.mv, c.live, jobs, out, substride, (int)os, scale, c.tw1, twc, tp_of<NC>());
+    SH_LAUNCHED();
+  }
+  static void syn(const FusedCtx &c, int, const float *in, int64_t substride, int64_t is, float2 *slabs,
+                  const PlaneJob *jobs, int njobs) {
+    if (njobs <= 0) return;
+    const int npl = OR == AX3 ? c.n3 : c.n2;
+    const float2 *twc = OR == AX3 ? c.tw2 : c.tw3;
+    hipLaunchKernelGGL((kQ_syn<N1, NC, OR>), dim3(npl, njobs), dim3(plane_threads<N1, NC>()), 0, c.stream, in,
+                       substride, (int)is, c.n1 * c.n2, c.mv, c.live, jobs, slabs, c.H, c.tw1, twc, tp_of<NC>());
+    SH_LAUNCHED();
+  }
+};
+
+template <int N1, int OR> bool set_plane(int nc, FusedOps &o) {
+  switch (nc) {
+    case 32: o.q_ana[OR] = &PlaneL<N1, 32, OR>::ana; o.q_syn[OR] = &PlaneL<N1, 32, OR>::syn; return true;
+    case 64: o.q_ana[OR] = &PlaneL<N1, 64, OR>::ana; o.q_syn[OR] = &PlaneL<N1, 64, OR>::syn; return true;
+    case 128: o.q_ana[OR] = &PlaneL<N1, 128, OR>::ana; o.q_syn[OR] = &PlaneL<N1, 128, OR>::syn; return true;
+    default: return false;
+  }
+}
+
+template <int OR> bool set_pencil(int np, FusedOps &o) {
+  switch (np) {
+    case 32: o.p_ana[OR] = &PencilL<32, OR>::ana; o.p_syn[OR] = &PencilL<32, OR>::syn; return true;
+    case 64: o.p_ana[OR] = &PencilL<64, OR>::ana; o.p_syn[OR] = &PencilL<64, OR>::syn; return true;
+    case 128: o.p_ana[OR] = &PencilL<128, OR>::ana; o.p_syn[OR] = &PencilL<128, OR>::syn; return true;
+    default: return false;
+  }
+}
+
+template <int N1> bool set_planes(int n2, int n3, FusedOps &o) {
+  return set_plane<N1, AX3>(n2, o) && set_plane<N1, AX2>(n3, o);
+}
+
+}  // namespace
+
+bool fused_ops_for(int n1, int n2, int n3, FusedOps &o) {
+  o = FusedOps{};
+  auto pow2 = [](int n) { return n == 32 || n == 64 || n == 128; };
+  if (!pow2(n1) || !pow2(n2) || !pow2(n3)) return false;
+  bool ok = set_pencil<AX3>(n3, o) && set_pencil<AX2>(n2, o);
+  switch (n1) {
+    case 32: ok = ok && set_planes<32>(n2, n3, o); break;
+    case 64: ok = ok && set_planes<64>(n2, n3, o); break;
+    default: ok = ok && set_planes<128>(n2, n3, o); break;
+  }
+  switch (n3) {
+    case 32: o.p_fwd = &launch_fwd<32>; break;
+    case 64: o.p_fwd = &launch_fwd<64>; break;
+    default: o.p_fwd = &launch_fwd<128>; break;
+  }
+  o.tile = tp_of<32>();
+  return ok;
+}
+
+}  // namespace surfhip

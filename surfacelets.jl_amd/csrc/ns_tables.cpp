@@ -175,17 +175,6 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
     if (empty)
       fail(SURFHIP_ERR_ARG, fmt("size (%d, %d, %d) is too small for %d scales: scale %d would be empty", n1, n2, n3, J, J - j));
   }
-  T.a.resize((size_t)(J + 1) * Lsum);
-  T.l.resize((size_t)J * Lsum);
-  T.c.resize((size_t)J * Lsum);
-  for (int j = 0; j <= J; ++j)
-    for (int k = 0; k < Lsum; ++k) T.a[(size_t)j * Lsum + k] = (float)acc[j][k];
-  for (int j = 0; j < J; ++j)
-    for (int k = 0; k < Lsum; ++k) {
-      T.l[(size_t)j * Lsum + k] = (float)lp[j][k];
-      T.c[(size_t)j * Lsum + k] = (float)(1.0 - lp[j][k]);
-    }
-
   // Hourglass tables: f0(M1) on the rfft grid, f0(M2) on (w2, w3).
   const int np1 = N + 1;
   auto b1 = bernstein_table(N, dft_frequencies(n1, true));
@@ -216,8 +205,7 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
       }
     }
   const size_t V = (size_t)h1 * n2 * n3;
-  T.hg1.resize(V);
-  T.hg1c.resize(V);
+  std::vector<double> hg1(V), hg1c(V), hg2((size_t)n2 * n3), hg2c((size_t)n2 * n3);
   std::vector<double> col(h1);
   for (int k3 = 0; k3 < n3; ++k3)
     for (int k2 = 0; k2 < n2; ++k2) {
@@ -231,12 +219,10 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
         double f0 = halfband_split(col[k1], poly);
         double f1 = halfband_split(-col[k1], poly);   // = 1 - f0, without cancellation
         size_t q = ((size_t)k1 * n2 + (size_t)k2) * n3 + k3;
-        T.hg1[q] = (float)(T.tight ? std::sqrt(std::max(f0, 0.0)) : f0);
-        T.hg1c[q] = (float)(T.tight ? std::sqrt(std::max(f1, 0.0)) : f1);
+        hg1[q] = T.tight ? std::sqrt(std::max(f0, 0.0)) : f0;
+        hg1c[q] = T.tight ? std::sqrt(std::max(f1, 0.0)) : f1;
       }
     }
-  T.hg2.resize((size_t)n2 * n3);
-  T.hg2c.resize((size_t)n2 * n3);
   for (int k3 = 0; k3 < n3; ++k3)
     for (int k2 = 0; k2 < n2; ++k2) {
       double m = 0.0;
@@ -246,8 +232,8 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
           if (c != 0.0) m += c * b2[(size_t)j2 * n2 + k2] * b3[(size_t)j3 * n3 + k3];
         }
       double f0 = halfband_split(m, poly), f1 = halfband_split(-m, poly);
-      T.hg2[(size_t)k2 * n3 + k3] = (float)(T.tight ? std::sqrt(std::max(f0, 0.0)) : f0);
-      T.hg2c[(size_t)k2 * n3 + k3] = (float)(T.tight ? std::sqrt(std::max(f1, 0.0)) : f1);
+      hg2[(size_t)k2 * n3 + k3] = T.tight ? std::sqrt(std::max(f0, 0.0)) : f0;
+      hg2c[(size_t)k2 * n3 + k3] = T.tight ? std::sqrt(std::max(f1, 0.0)) : f1;
     }
 
   // Wedge planes, one pool; each plane is stored over its two axes in
@@ -257,6 +243,8 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
   T.items[0] = NSItem{1, J, 0, -1, 0, -1, 0, -1};
   T.group_first.clear();
   T.group_count.clear();
+  T.group_na.clear();
+  T.group_nb.clear();
   int offset = 1;  // 0-based subband index of scale j's first subband
   for (int j = 1; j <= J; ++j) {
     const int Lj = p.direction_levels[j - 1];
@@ -294,6 +282,8 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
       const int g = 3 * (j - 1) + (i - 1);
       T.group_first.push_back(offset + (i - 1) * count[0] * count[1]);
       T.group_count.push_back(count[0] * count[1]);
+      T.group_na.push_back(count[0]);
+      T.group_nb.push_back(count[1]);
       for (int a = 0; a < count[0]; ++a)
         for (int b = 0; b < count[1]; ++b) {
           int k = offset + ((i - 1) * count[0] + a) * count[1] + b;  // _ns_index, 0-based
@@ -301,6 +291,43 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
         }
     }
     offset += 3 * (1 << Lj);
+  }
+
+  // Group factors in FP64, rounded once (src/transforms/ns_surfacelet.jl:
+  // 205-208,226-227,265-272): group g = 3 (j - 1) + (i - 1) has
+  // B = a1 a2 a3 sqrt(max(1 - (l1 l2 l3)^2, 0)) at pyramid level J - j times
+  // the hourglass weight W_i = (W1, W2, W3) = (f0(M1), f1(M1) f0(M2),
+  // f1(M1) f1(M2)); the lowpass (last) is a1 a2 a3 at level J.
+  const int G = 3 * J;
+  T.ngroups = G + 1;
+  T.gana.assign((size_t)T.ngroups * V, 0.0f);
+  if (!T.tight) T.gsyn.assign((size_t)T.ngroups * V, 0.0f);
+  else T.gsyn.clear();
+  for (int g = 0; g <= G; ++g) {
+    const bool low = g == G;
+    const int lev = low ? J : J - (g / 3 + 1);
+    const int i = g % 3 + 1;
+    const double *ax = acc[lev].data();
+    const double *lx = low ? nullptr : lp[lev].data();
+    float *ga = T.gana.data() + (size_t)g * V;
+    float *gs = T.tight ? nullptr : T.gsyn.data() + (size_t)g * V;
+    for (int k1 = 0; k1 < h1; ++k1)
+      for (int k2 = 0; k2 < n2; ++k2)
+        for (int k3 = 0; k3 < n3; ++k3) {
+          const size_t q = ((size_t)k1 * n2 + k2) * n3 + k3;
+          const double a3 = ax[k1] * ax[h1 + k2] * ax[h1 + n2 + k3];
+          if (low) {
+            ga[q] = (float)a3;
+            if (gs) gs[q] = (float)a3;
+            continue;
+          }
+          const double L = lx[k1] * lx[h1 + k2] * lx[h1 + n2 + k3];
+          const double B = a3 * std::sqrt(std::max(1.0 - L * L, 0.0));
+          const size_t q2 = (size_t)k2 * n3 + k3;
+          const double W = i == 1 ? hg1[q] : hg1c[q] * (i == 2 ? hg2[q2] : hg2c[q2]);
+          ga[q] = (float)(B * W);
+          if (gs) gs[q] = (float)B;
+        }
   }
 }
 

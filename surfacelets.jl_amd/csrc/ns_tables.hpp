@@ -1,5 +1,5 @@
 // NSSurfacelet mask tables (built on the host in FP64, stored as FP32) and
-// the per-voxel mask expression the kernels evaluate on the fly.
+// the per-voxel mask expression the kernels evaluate from them.
 //
 // Reference: src/transforms/ns_surfacelet.jl:98-123 (tables), :222-242 (the
 // mask bank), :265-272 (bandpass and hourglass factors), :295-303 (group
@@ -28,15 +28,15 @@ struct NSItem {
 
 // Raw table pointers (host or device) and the grid.
 struct NSMaskView {
-  // All tables are C-ordered over (k1, k2, k3) -- k3 fastest -- so that a
-  // pencil along axis 3 reads them contiguously.
-  const float *hg1, *hg1c;  // [k1][k2][k3]: stage-1 factor f0(M1) and f1(M1), sqrt'd when tight
-  const float *hg2, *hg2c;  // [k2][k3]: stage-2 factor f0(M2) and f1(M2), sqrt'd when tight
-  const float *a;           // (J+1) x (h1+n2+n3): prod_{i<j} l_i per axis
-  const float *l;           // J x (h1+n2+n3): l_j per axis
-  const float *c;           // J x (h1+n2+n3): 1 - l_j per axis (formed in FP64)
+  // Group factors, one rfft-grid table per group and the lowpass last,
+  // [g][k1][k2][k3] (k3 fastest, so a pencil along axis 3 reads contiguously):
+  //   analysis: G = B_j W_i (the bandpass of the group's pyramid level times
+  //   its hourglass weight, both square-rooted when tight); lowpass a1 a2 a3;
+  //   synthesis: the same when tight, B_j alone otherwise.
+  // Each value is formed in FP64 and rounded once.
+  const float *gana, *gsyn;
   const float *planes;      // wedge plane pool (sqrt'd when tight), each [lower axis][higher axis]
-  int32_t h1, n2, n3, J, tight;
+  int32_t h1, n2, n3, ngroups, tight;
   // Mask swaps for the adjoints: every subband operator irfft . M . rfft is
   // self-adjoint (M real and even), so A^T is the accumulation with the
   // analysis masks and S^T the analysis with the synthesis masks.  Only a
@@ -51,21 +51,15 @@ SH_HD inline float ns_plane(const NSMaskView &v, int32_t off, int32_t pair, int 
   return v.planes[off + idx];
 }
 
-// The bandpass of pyramid level `lev` at (k1, k2, k3):
-// B = a1 a2 a3 sqrt(max(1 - (l1 l2 l3)^2, 0)) (src/transforms/ns_surfacelet.jl:271-272).
-// 1 - L is formed as c1 + l1 c2 + l1 l2 c3 from the FP64 complements, which
-// is exact algebra and avoids the FP32 cancellation of 1 - L near L = 1.
-SH_HD inline float ns_bandpass(const NSMaskView &v, int lev, int k1, int k2, int k3) {
-  const int L = v.h1 + v.n2 + v.n3;
-  const float *a = v.a + (size_t)lev * L;
-  const float *l = v.l + (size_t)lev * L;
-  const float *c = v.c + (size_t)lev * L;
-  const int i2 = v.h1 + k2, i3 = v.h1 + v.n2 + k3;
-  float l1 = l[k1], l2 = l[i2], l3 = l[i3];
-  float om = c[k1] + l1 * c[i2] + (l1 * l2) * c[i3];
-  float op = 1.0f + (l1 * l2) * l3;
-  float r = om * op;
-  return (a[k1] * a[i2] * a[i3]) * sqrtf(r > 0.0f ? r : 0.0f);
+// Table index of group `g` of the item (the lowpass is the last group).
+SH_HD inline int ns_group_index(const NSMaskView &v, const NSItem &it) {
+  return it.lowpass ? v.ngroups - 1 : it.group;
+}
+
+// The group-factor table of group g for analysis or synthesis masks.
+SH_HD inline const float *ns_group_table(const NSMaskView &v, int g, bool analysis) {
+  const bool ana = analysis ? !v.ana_uses_syn : v.syn_uses_ana;
+  return (ana ? v.gana : v.gsyn) + (size_t)g * ((size_t)v.h1 * v.n2 * v.n3);
 }
 
 // Group factor of subband `it` at rfft bin (k1, k2, k3): B_j times the
@@ -73,23 +67,31 @@ SH_HD inline float ns_bandpass(const NSMaskView &v, int lev, int k1, int k2, int
 // synthesis of a bank that is not tight), or the lowpass a1 a2 a3
 // (src/transforms/ns_surfacelet.jl:226-227,265-272,295-303,350-358).
 SH_HD inline float ns_group(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
-  if (it.lowpass) {
-    const int L = v.h1 + v.n2 + v.n3;
-    const float *a = v.a + (size_t)v.J * L;
-    return a[k1] * a[v.h1 + k2] * a[v.h1 + v.n2 + k3];
-  }
-  float B = ns_bandpass(v, it.level, k1, k2, k3);
-  if (!v.tight && (analysis ? v.ana_uses_syn : !v.syn_uses_ana)) return B;
-  size_t p3 = ((size_t)k1 * v.n2 + (size_t)k2) * v.n3 + k3;
-  size_t p2 = (size_t)k2 * v.n3 + k3;
-  float h = it.hourglass == 1 ? v.hg1[p3] : v.hg1c[p3] * (it.hourglass == 2 ? v.hg2[p2] : v.hg2c[p2]);
-  return B * h;
+  return ns_group_table(v, ns_group_index(v, it), analysis)[((size_t)k1 * v.n2 + k2) * v.n3 + k3];
+}
+
+// Whether the wedge factor is the unit mask: the synthesis of a bank that is
+// not tight (src/transforms/ns_surfacelet.jl:226-238).
+SH_HD inline bool ns_unit_wedge(const NSMaskView &v, bool analysis) {
+  return !v.tight && (analysis ? v.ana_uses_syn : !v.syn_uses_ana);
+}
+
+// The two wedge planes of subband `it`: w_a over (hourglass axis, first
+// transverse axis), w_b over (hourglass axis, second transverse axis); 1 for
+// the lowpass and for a unit wedge.
+SH_HD inline float ns_wedge_a(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
+  if (it.lowpass || ns_unit_wedge(v, analysis)) return 1.0f;
+  return ns_plane(v, it.pa_off, it.pa_pair, k1, k2, k3);
+}
+SH_HD inline float ns_wedge_b(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
+  if (it.lowpass || ns_unit_wedge(v, analysis)) return 1.0f;
+  return ns_plane(v, it.pb_off, it.pb_pair, k1, k2, k3);
 }
 
 // Wedge factor w_a w_b (src/transforms/ns_surfacelet.jl:376-427); 1 for the
 // lowpass and for the synthesis of a bank that is not tight (unit wedge).
 SH_HD inline float ns_wedge(const NSMaskView &v, const NSItem &it, int k1, int k2, int k3, bool analysis) {
-  if (it.lowpass || (!v.tight && (analysis ? v.ana_uses_syn : !v.syn_uses_ana))) return 1.0f;
+  if (it.lowpass || ns_unit_wedge(v, analysis)) return 1.0f;
   return ns_plane(v, it.pa_off, it.pa_pair, k1, k2, k3) * ns_plane(v, it.pb_off, it.pb_pair, k1, k2, k3);
 }
 
@@ -103,12 +105,15 @@ SH_HD inline float ns_mask(const NSMaskView &v, const NSItem &it, int k1, int k2
 // Host-side tables of one (transform, size).
 struct NSTables {
   int32_t n1, n2, n3, h1, J, S, tight;
-  std::vector<float> hg1, hg1c, hg2, hg2c, a, l, c, planes;
+  int32_t ngroups;                 // 3 J groups and the lowpass
+  std::vector<float> gana, gsyn;   // ngroups rfft-grid tables each; gsyn empty when tight (= gana)
+  std::vector<float> planes;
   std::vector<NSItem> items;  // S entries, subband_layout order
   std::vector<int32_t> group_first, group_count;  // items of each group (subband ranges)
+  std::vector<int32_t> group_na, group_nb;        // wedge counts of planes a and b per group
   NSMaskView view() const {
-    return NSMaskView{hg1.data(), hg1c.data(), hg2.data(), hg2c.data(), a.data(), l.data(),
-                      c.data(), planes.data(), h1, n2, n3, J, tight, 0, 0};
+    const float *gs = gsyn.empty() ? gana.data() : gsyn.data();
+    return NSMaskView{gana.data(), gs, planes.data(), h1, n2, n3, ngroups, tight, 0, 0};
   }
 };
 

@@ -202,3 +202,23 @@ def test_ns_128_default_transform_matches_oracle():
     y2 = torch.empty_like(xr)
     N.synthesize_(lambda i: N.subband(t, c, i), y2, t, workspace=ws)
     assert torch.equal(xr, y2)
+
+
+@pytest.mark.parametrize("dims,J,L,tight", [((64, 32, 128), 3, (1, 2, 3), True), ((32, 64, 64), 2, (2, 3), False)])
+def test_ns_tile_pruning_changes_nothing(dims, J, L, tight, monkeypatch):
+    """Tiles whose masks are exactly zero are skipped; the results must equal
+    those of the plan that transforms every tile, bit for bit."""
+    dev = _dev()
+    print(f"case dims={dims} J={J} L={L} tight={tight}", flush=True)
+    rng = np.random.default_rng(97)
+    x = from_numpy(rng.standard_normal(dims).astype(np.float32), dev)
+    t = N.NSSurfacelet(scales=J, direction_levels=list(L), bernstein_order=12, tight=tight)
+    ws = N.make_workspace(t, x)
+    monkeypatch.setenv("SURFHIP_NS_NO_PRUNE", "1")
+    ws_full = N.make_workspace(t, x)
+    monkeypatch.delenv("SURFHIP_NS_NO_PRUNE")
+    c = N.analyze(t, x, workspace=ws)
+    c_full = N.analyze(t, x, workspace=ws_full)
+    assert torch.equal(c, c_full)
+    r = from_numpy(rng.standard_normal(tuple(dims) + (N.nsubbands(t, dims),)).astype(np.float32), dev)
+    assert torch.equal(N.synthesize(t, r, workspace=ws), N.synthesize(t, r, workspace=ws_full))
