@@ -56,6 +56,16 @@ template <int N1, int NC> constexpr int plane_threads() {
 // consecutive slots each thread reads in phase 2 start in different banks.
 template <int R1> __device__ __forceinline__ int xpos(int e) { return e + e / R1; }
 
+// Synchronisation of the lanes of one wave around LDS traffic that only they
+// share (the exchange of a pencil whose threads are consecutive lanes): a
+// wave executes its LDS instructions in order, so ordering the compiler's
+// view of memory is enough; no block barrier is needed.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // A copy of v the compiler cannot see through: index math derived from it
 // inside a loop is recomputed per iteration instead of being hoisted and kept
 // live (in registers) across the loop.
@@ -458,7 +468,8 @@ template <int N1, int NC, int OR>
 __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     const float2 *__restrict__ slabs, int64_t H, int n1n2, NSMaskView mv, const uint32_t *__restrict__ live,
     const PlaneJob *__restrict__ jobs, float *__restrict__ out, int64_t substride, int os, float scale,
-    const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg, int tpw) {
+    const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg) {
+  constexpr int tpw = tp_of<NC>();  // columns per live-tile bit
   constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = pencil_stride<NC>(), PP = pencil_stride<N1>();
   constexpr int CS = col_stride<N1>();
   constexpr int NTH = ana_threads<N1, NC>();
@@ -504,18 +515,20 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
 #pragma unroll
       for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + j)] = v[s];
     }
-    __syncthreads();
+    wave_sync();  // the exchange region of a row is private to its R2 lanes
     if (act) {
 #pragma unroll
       for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(j * RC1 + e)];
       RegFFT<NC, 1>::phase2(v);
     }
     __syncthreads();
-    if (act) {
+    if (onslot) {
+      // Rows without live tiles hold zeros.
 #pragma unroll
       for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
-        for (int qq = 0; qq < RC2; ++qq) L[((j * RCR + uu) + RC1 * qq) * CS + slot] = v[uu * RC2 + qq];
+        for (int qq = 0; qq < RC2; ++qq)
+          L[((j * RCR + uu) + RC1 * qq) * CS + slot] = act ? v[uu * RC2 + qq] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
   }
@@ -524,6 +537,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
   // that hold no data are zero.
   using F = RegFFT<N1, 1>;
   constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS;
+  static_assert(R1 % 2 == 0 && R1 * R2 == 2 * HS, "row HS sits at element R1 / 2 of thread 0");
   const int pr = tid / R2, jj = tid % R2;
   const bool on = pr < NC / 2;
   float2 w[R1];
@@ -531,19 +545,20 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     const float2 *ca = L + (2 * pr) * CS, *cb = ca + CS;
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
+      // Row t = jj + R2 r: below HS = R1 R2 / 2 for r < R1 / 2, above it
+      // (the mirror N1 - t, conjugated) for r > R1 / 2; r = R1 / 2 holds HS
+      // on thread jj = 0 and mirrors elsewhere.  Rows 0 and HS are real.
       const int t = jj + R2 * r;
-      const int tt = t <= HS ? t : N1 - t;
-      float2 A = make_float2(0.0f, 0.0f), B = A;
-      if (rl[tt]) {
-        A = ca[tt];
-        B = cb[tt];
-      }
-      if (t == 0 || t == HS) {
-        A.y = 0.0f;
-        B.y = 0.0f;
-      } else if (t > HS) {
+      const bool mir = 2 * r > R1 || (2 * r == R1 && jj > 0);
+      const bool real = (r == 0 || 2 * r == R1) && jj == 0;
+      float2 A = ca[mir ? N1 - t : t], B = cb[mir ? N1 - t : t];
+      if (mir) {
         A = conjf2(A);
         B = conjf2(B);
+      }
+      if (real) {
+        A.y = 0.0f;
+        B.y = 0.0f;
       }
       w[r] = make_float2(A.x - B.y, A.y + B.x);
     }
@@ -555,7 +570,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
 #pragma unroll
     for (int s = 0; s < R1; ++s) qb[xpos<R1>(s * R2 + jj)] = w[s];
   }
-  __syncthreads();
+  wave_sync();  // the exchange region of a column pair is private to its R2 lanes
   if (!on) return;
 #pragma unroll
   for (int e = 0; e < R1; ++e) w[e] = qb[xpos<R1>(jj * R1 + e)];
@@ -618,11 +633,13 @@ template <int N1, int NC, int OR>
 __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
     const float *__restrict__ in, int64_t substride, int is, int n1n2, NSMaskView mv,
     const uint32_t *__restrict__ live, const PlaneJob *__restrict__ jobs, float2 *__restrict__ slabs, int64_t H,
-    const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg, int tpw) {
+    const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg) {
+  constexpr int tpw = tp_of<NC>();  // columns per live-tile bit
   constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = pencil_stride<NC>(), PP = pencil_stride<N1>();
   constexpr int NTH = plane_threads<N1, NC>();
   constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
   constexpr int C1 = FourStep<N1>::R1, C2 = FourStep<N1>::R2, CR = C1 / C2;
+  static_assert(C2 % 2 == 0 && C1 * C2 == 2 * HS, "row HS sits at element (0, C2 / 2) of thread 0");
   constexpr int NLD = N1 * NC / 2 / NTH;  // column-pair elements per thread
   __shared__ __align__(16) float2 L[plane_slots<N1, NC>()];
   __shared__ float2 tw1[N1], twc[NC], R64[NC];
@@ -676,17 +693,18 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
       const bool on = pr < NC / 2;
       float2 *qb = L + (on ? pr : 0) * PP;
       float2 v[C1];
+      // The exchange region of a column pair is private to its C2 lanes.
       if (on) {
 #pragma unroll
         for (int r = 0; r < C1; ++r) v[r] = qb[jj + C2 * r];
         RegFFT<N1, -1>::phase1(v, jj, tw1);
       }
-      __syncthreads();
+      wave_sync();
       if (on) {
 #pragma unroll
         for (int s = 0; s < C1; ++s) qb[xpos<C1>(s * C2 + jj)] = v[s];
       }
-      __syncthreads();
+      wave_sync();
       if (on) {
 #pragma unroll
         for (int e = 0; e < C1; ++e) v[e] = qb[xpos<C1>(jj * C1 + e)];
@@ -694,28 +712,27 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
       }
       __syncthreads();
       {
-        float2 z0 = make_float2(0.0f, 0.0f), zh = z0;
+        // Row k = (jj CR + u) + C1 q lies below HS = C1 C2 / 2 for every
+        // q < C2 / 2; row HS is element (0, C2 / 2) of thread jj = 0.  Rows
+        // without live tiles are written too: the row pass skips them.
 #pragma unroll
         for (int u = 0; u < CR; ++u)
 #pragma unroll
-          for (int q = 0; q < C2; ++q) {
-            const int k = (jj * CR + u) + C1 * q, m = u * C2 + q;
-            // Rows k <= HS need the mirror; with C1 q > HS no lane does.
-            if (C1 * q > HS) continue;
+          for (int q = 0; q < C2 / 2; ++q) {
             const float2 Zm = fft_mirror<N1>(v, jj, u, q);
-            if (!on || k > HS) continue;
-            if (k == 0) z0 = v[m];
-            if (k == HS) zh = v[m];
-            if (k == 0 || k == HS || !rl[k]) continue;
-            const float2 Z = v[m], Zc = conjf2(Zm);
-            const float2 d = csub(Z, Zc);
-            L[k * PS + 2 * pr] = cscale(cadd(Z, Zc), 0.5f);
-            L[k * PS + 2 * pr + 1] = make_float2(0.5f * d.y, -0.5f * d.x);
+            if (on) {
+              const float2 Z = v[u * C2 + q], Zc = conjf2(Zm);
+              const float2 d = csub(Z, Zc);
+              float2 *lc = L + (jj * CR + u + C1 * q) * PS + 2 * pr;
+              lc[0] = cscale(cadd(Z, Zc), 0.5f);
+              lc[1] = make_float2(0.5f * d.y, -0.5f * d.x);
+            }
           }
-        // Thread jj = 0 holds k = 0 and k = HS (HS is a multiple of C1).
+        // Rows 0 and HS are real: slot 0 takes row 0 + i row HS (replacing
+        // the row 0 pair written above by the same thread).
         if (on && jj == 0) {
-          L[2 * pr] = make_float2(z0.x, zh.x);
-          L[2 * pr + 1] = make_float2(z0.y, zh.y);
+          L[2 * pr] = make_float2(v[0].x, v[C2 / 2].x);
+          L[2 * pr + 1] = make_float2(v[0].y, v[C2 / 2].y);
         }
       }
     }
@@ -724,23 +741,25 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
       const int sl = t / RC2, jr = t % RC2;
       float2 *pb = L + (onslot ? sl : 0) * PS;
       float2 v[RC1];
+      // The exchange region of a row is private to its RC2 lanes.
       if (act) {
 #pragma unroll
         for (int r = 0; r < RC1; ++r) v[r] = pb[jr + RC2 * r];
         RegFFT<NC, -1>::phase1(v, jr, twc);
       }
-      __syncthreads();
+      wave_sync();
       if (act) {
 #pragma unroll
         for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + jr)] = v[s];
       }
-      __syncthreads();
+      wave_sync();
       if (act) {
 #pragma unroll
         for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(jr * RC1 + e)];
         RegFFT<NC, -1>::phase2(v);
       }
-      const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
+      // Plane a of subband p, from row sl and column jr RCR (unit: no table).
+      const float *wrow = mv.planes + (wunit ? 0 : jb.pa0 + (p - jb.p0) * jb.pa_step + sl * NC + jr * RCR);
       if (slot == 0) {
         // Z = F(row 0) + i F(row HS), both rows real: split through Z[-kc].
 #pragma unroll
@@ -754,20 +773,22 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
               const float2 F0 = cscale(cadd(Z, Zc), 0.5f);
               const float2 d = csub(Z, Zc);
               const float2 F1 = make_float2(0.5f * d.y, -0.5f * d.x);
-              const float w0 = wab[wunit ? 0u : (unsigned)kc];
-              const float w1 = wab[wunit ? 0u : (unsigned)(HS * NC + kc)];
-              R[m] = fma2(F0, wunit ? 1.0f : w0, R[m]);
-              R64[kc] = fma2(F1, wunit ? 1.0f : w1, R64[kc]);
+              const float w0 = wunit ? 1.0f : wrow[uu + RC1 * qq];
+              const float w1 = wunit ? 1.0f : wrow[HS * NC + uu + RC1 * qq];
+              R[m] = fma2(F0, w0, R[m]);
+              R64[kc] = fma2(F1, w1, R64[kc]);
             }
-        }
+          }
+      } else if (act && wunit) {
+#pragma unroll
+        for (int m = 0; m < RC1; ++m) R[m] = cadd(v[m], R[m]);
       } else if (act) {
 #pragma unroll
         for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
           for (int qq = 0; qq < RC2; ++qq) {
-            const int kc = (jr * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
-            const float wv = wab[wunit ? 0u : (unsigned)(sl * NC + kc)];
-            R[m] = fma2(v[m], wunit ? 1.0f : wv, R[m]);
+            const int m = uu * RC2 + qq;
+            R[m] = fma2(v[m], wrow[uu + RC1 * qq], R[m]);
           }
       }
     }
@@ -820,7 +841,7 @@ template <int N1, int NC, int OR> struct PlaneL {
     const int npl = OR == AX3 ? c.n3 : c.n2;
     const float2 *twc = OR == AX3 ? c.tw2 : c.tw3;
     hipLaunchKernelGGL((kQ_ana<N1, NC, OR>), dim3(npl, njobs), dim3(ana_threads<N1, NC>()), 0, c.stream, slabs, c.H,
-                       c.n1 * c.n2, c.mv, c.live, jobs, out, substride, (int)os, scale, c.tw1, twc, tp_of<NC>());
+                       c.n1 * c.n2, c.mv, c.live, jobs, out, substride, (int)os, scale, c.tw1, twc);
     SH_LAUNCHED();
   }
   static void syn(const FusedCtx &c, int, const float *in, int64_t substride, int64_t is, float2 *slabs,
@@ -829,7 +850,7 @@ template <int N1, int NC, int OR> struct PlaneL {
     const int npl = OR == AX3 ? c.n3 : c.n2;
     const float2 *twc = OR == AX3 ? c.tw2 : c.tw3;
     hipLaunchKernelGGL((kQ_syn<N1, NC, OR>), dim3(npl, njobs), dim3(plane_threads<N1, NC>()), 0, c.stream, in,
-                       substride, (int)is, c.n1 * c.n2, c.mv, c.live, jobs, slabs, c.H, c.tw1, twc, tp_of<NC>());
+                       substride, (int)is, c.n1 * c.n2, c.mv, c.live, jobs, slabs, c.H, c.tw1, twc);
     SH_LAUNCHED();
   }
 };
