@@ -81,11 +81,19 @@ template <int DIR> __device__ __forceinline__ float2 tmul(float2 v, const float2
 
 template <int N, int DIR> struct RegFFT {
   static constexpr int R1 = FourStep<N>::R1, R2 = FourStep<N>::R2, ROWS = R1 / R2;
-  // Phase 1 on values already in registers (v[r] = x[j + R2 r]).
-  __device__ static __forceinline__ void phase1(float2 (&v)[R1], int j, const float2 *tw) {
+  // Phase 1 on values already in registers (v[r] = x[j + R2 r]).  twT is the
+  // twiddle table in phase-1 order (fill_twiddles): thread j reads
+  // twT[s R2 + j] = w_N^(j s), one base address and constant offsets.
+  __device__ static __forceinline__ void phase1(float2 (&v)[R1], int j, const float2 *twT) {
     dftR<R1, DIR>(v);
+    const float2 *tj = twT + j;
 #pragma unroll
-    for (int s = 1; s < R1; ++s) v[s] = tmul<DIR>(v[s], tw, j * s);
+    for (int s = 1; s < R1; ++s) v[s] = tmul<DIR>(v[s], tj, s * R2);
+  }
+  // twT[s R2 + j] = tw[j s] from the N forward twiddles tw[m] =
+  // exp(-2 pi i m / N); j s < N.  Threads t = first, first + step, ...
+  __device__ static __forceinline__ void fill_twiddles(float2 *twT, const float2 *tw, int first, int step) {
+    for (int m = first; m < N; m += step) twT[m] = tw[(m % R2) * (m / R2)];
   }
   // Phase 2 on v[u R2 + q'] = slot((j ROWS + u) R2 + q'); results in place:
   // v[u R2 + q] = X[(j ROWS + u) + R1 q].

@@ -216,13 +216,15 @@ struct surfhip_ns_plan {
   std::vector<PlaneJob> jobs;       // [analysis AX3, AX2][synthesis AX3, AX2][whole grid][S fresh][S continuing]
   int ja_ax[2] = {0, 0}, jna_ax[2] = {0, 0}, js_ax[2] = {0, 0}, jns_ax[2] = {0, 0};
   int j_full = 0, j_fresh = 0, j_cont = 0;
-  std::vector<int32_t> glist;       // synthesis group order: AX2 groups, then AX3 groups + lowpass
-  int gl_beg[2] = {0, 0}, gl_n[2] = {0, 0};
+  // Synthesis slab order {slab, group, item of q, 0}: AX2 groups, then AX3
+  // groups and the lowpass, q ascending within a group.
+  std::vector<int4> slist;
+  int sl_beg[2] = {0, 0}, sl_n[2] = {0, 0};
   DevBuf<GroupDev> dgroups;
   DevBuf<uint32_t> dlive;
   DevBuf<int4> dunits;
   DevBuf<PlaneJob> djobs;
-  DevBuf<int32_t> dglist;
+  DevBuf<int4> dslist;
   DevBuf<float2> U[2], R[2], TV, accf;
   int cached_group = -1;            // streaming analysis: the group whose slabs U holds
   std::vector<char> touched;        // streaming synthesis: slabs holding a partial sum
@@ -425,18 +427,19 @@ struct surfhip_ns_plan {
       }
     }
     // Synthesis group order.
-    glist.clear();
+    slist.clear();
     for (int o : {AX2, AX3}) {
-      gl_beg[o] = (int)glist.size();
+      sl_beg[o] = (int)slist.size();
       for (int g = 0; g <= G; ++g)
-        if (groups[g].orient == o) glist.push_back(g);
-      gl_n[o] = (int)glist.size() - gl_beg[o];
+        if (groups[g].orient == o)
+          for (int q = 0; q < groups[g].nB; ++q) slist.push_back(make_int4(groups[g].slab0 + q, g, groups[g].first + q, 0));
+      sl_n[o] = (int)slist.size() - sl_beg[o];
     }
     dgroups.upload(groups);
     dlive.upload(live);
     dunits.upload(units);
     djobs.upload(jobs);
-    dglist.upload(glist);
+    dslist.upload(slist);
     for (int ch = 0; ch < nch; ++ch) {
       U[ch].alloc((size_t)nslabs * H);
       R[ch].alloc((size_t)nslabs * H);
@@ -549,13 +552,13 @@ struct surfhip_ns_plan {
   // Close the synthesis sum of channel ch from its slabs and write the volume.
   void finish_channel(int ch, float *x) {
     const FusedCtx c = ctx();
-    const bool has2 = gl_n[AX2] > 0;
+    const bool has2 = sl_n[AX2] > 0;
     if (has2)
       timed(3, [&] {
-        fops.p_syn[AX2](c, AX2, R[ch].p, dglist.p + gl_beg[AX2], gl_n[AX2], accf.p, 0, nullptr);
+        fops.p_syn[AX2](c, AX2, R[ch].p, dslist.p + sl_beg[AX2], sl_n[AX2], accf.p, 0, nullptr);
       });
     timed(3, [&] {
-      fops.p_syn[AX3](c, AX3, R[ch].p, dglist.p + gl_beg[AX3], gl_n[AX3], accf.p, has2 ? 1 : 0, TV.p);
+      fops.p_syn[AX3](c, AX3, R[ch].p, dslist.p + sl_beg[AX3], sl_n[AX3], accf.p, has2 ? 1 : 0, TV.p);
     });
     timed(1, [&] { fops.q_ana[AX3](c, AX3, TV.p, djobs.p + j_full, 1, x + ch, 0, nch, 1.0f / (float)N); });
   }

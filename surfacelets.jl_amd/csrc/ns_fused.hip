@@ -182,11 +182,11 @@ template <int NP> constexpr int pblock_threads() { return 2 * half_threads<NP>()
 // that a warp touches contiguous memory of the pencil-major spectrum: along
 // the pencil for AX3 (pencils are contiguous), across the tile for AX2.
 // `off` is the element's offset inside the k1 slab of an [k1][k2][k3] array.
-template <int NP, int OR> struct TileElem {
+template <int NP, int OR, int NTH = pblock_threads<NP>()> struct TileElem {
   int pp, kp, k2, k3;
   unsigned off;
   __device__ __forceinline__ TileElem(int tid, int m, int c0, int n3) {
-    constexpr int TP = tp_of<NP>(), NTH = pblock_threads<NP>();
+    constexpr int TP = tp_of<NP>();
     const int e = tid + NTH * m;
     if (OR == AX3) {
       pp = e / NP;
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *
   const int g = un.x, k1 = un.y, c0 = un.z, tile = c0 / TP;
   const GroupDev gd = groups[g];
   const int h1 = mv.h1, nt = OR == AX3 ? n2 : n3;
-  for (int m = tid; m < NP; m += NTH) tw[m] = twg[m];
+  RegFFT<NP, 1>::fill_twiddles(tw, twg, tid, NTH);
   const size_t kb = (size_t)k1 * n2 * n3;
   const float2 *Xk = X + kb;
   const float *Gk = ns_group_table(mv, g, true) + kb;
@@ -292,104 +292,110 @@ __global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *
 
 // Synthesis pencil pass over one tile (k1, c0) of orientation OR:
 //   a = acc (AX3 with has_acc) or 0;
-//   for g in glist, for each live q: a = fma(FFT_pencil(slab_q), G_g w_b,q, a)
-// in that order.  AX2 writes a into acc; AX3 writes IFFT_3(a) into V
-// ([i3][k1][k2]).
+//   for each slab s of slist, in order, whose tile is live:
+//     a = fma(FFT_pencil(slab_s), G_g(s) w_b(s), a)
+// AX2 writes a into acc; AX3 writes IFFT_3(a) into V ([i3][k1][k2]).  The
+// NQ quarters of the block transform NQ consecutive slabs of the list at a
+// time; the sum still runs over them in list order.
+template <int NP> constexpr int syn_quarters() { return 4; }
+template <int NP> constexpr int syn_threads() { return syn_quarters<NP>() * half_threads<NP>(); }
+
 template <int NP, int OR>
-__global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_syn(const float2 *__restrict__ slabs, int64_t H,
-                                                                  int n2, int n3, NSMaskView mv,
-                                                                  const NSItem *__restrict__ items,
-                                                                  const GroupDev *__restrict__ groups,
-                                                                  const uint32_t *__restrict__ live,
-                                                                  const int32_t *__restrict__ glist, int ng,
-                                                                  float2 *__restrict__ acc, int has_acc,
-                                                                  float2 *__restrict__ V,
-                                                                  const float2 *__restrict__ twg) {
-  constexpr int TP = tp_of<NP>(), NTH = pblock_threads<NP>(), HT = half_threads<NP>(), PS = pencil_stride<NP>();
+__global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__restrict__ slabs, int64_t H,
+                                                               int n2, int n3, NSMaskView mv,
+                                                               const NSItem *__restrict__ items,
+                                                               const uint32_t *__restrict__ live,
+                                                               const int4 *__restrict__ slist, int ns,
+                                                               float2 *__restrict__ acc, int has_acc,
+                                                               float2 *__restrict__ V,
+                                                               const float2 *__restrict__ twg) {
+  constexpr int TP = tp_of<NP>(), NQ = syn_quarters<NP>(), NTH = syn_threads<NP>(), QT = half_threads<NP>();
+  constexpr int PS = pencil_stride<NP>();
   constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2, ROWS = R1 / R2, E = TP * NP / NTH;
-  __shared__ float2 buf[2][TP * PS];
+  using TE = TileElem<NP, OR, NTH>;
+  __shared__ float2 buf[NQ][TP * PS];
   __shared__ float2 tw[NP];
   const int tid = threadIdx.x;
   const int h1 = mv.h1, nt = OR == AX3 ? n2 : n3, ntiles = nt / TP;
   const int k1 = blockIdx.x / ntiles, tile = blockIdx.x % ntiles, c0 = tile * TP;
-  for (int m = tid; m < NP; m += NTH) tw[m] = twg[m];
+  RegFFT<NP, 1>::fill_twiddles(tw, twg, tid, NTH);
   const size_t kb = (size_t)k1 * n2 * n3;
   float2 a[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const TileElem<NP, OR> t(tid, m, c0, n3);
+    const TE t(tid, m, c0, n3);
     a[m] = (OR == AX3 && has_acc) ? acc[kb + t.off] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
-  const int half = tid / HT, lt = tid % HT, pq = lt % TP, j = lt / TP;
+  const int quarter = tid / QT, lt = tid % QT, pq = lt % TP, j = lt / TP;
   const int rs = h1 * nt;
-  for (int gi = 0; gi < ng; ++gi) {
-    const int g = glist[gi];
-    const GroupDev gd = groups[g];
-    const float *Gk = ns_group_table(mv, g, false) + kb;
-    for (int q0 = 0; q0 < gd.nB; q0 += 2) {
-      bool lv[2];
+  for (int s0 = 0; s0 < ns; s0 += NQ) {
+    bool lv[NQ];
+    bool any = false;
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        lv[h] = q0 + h < gd.nB && ((live[(size_t)(gd.slab0 + q0 + h) * h1 + k1] >> tile) & 1u);
-      if (!lv[0] && !lv[1]) continue;
-      const bool act = lv[half];
-      float2 v[R1];
-      if (act) {
-        const float2 *src = slabs + (size_t)(gd.slab0 + q0 + half) * H + (size_t)k1 * nt + c0;
-#pragma unroll
-        for (int r = 0; r < R1; ++r) v[r] = src[(unsigned)((j + R2 * r) * rs + pq)];
-      }
-      float2 *pb = buf[half] + pq * PS;
-      if (act) {
-        RegFFT<NP, -1>::phase1(v, j, tw);
-#pragma unroll
-        for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
-      }
-      __syncthreads();
-      if (act) {
-#pragma unroll
-        for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
-        RegFFT<NP, -1>::phase2(v);
-      }
-      __syncthreads();
-      if (act) {
-#pragma unroll
-        for (int uu = 0; uu < ROWS; ++uu)
-#pragma unroll
-          for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (!lv[h]) continue;
-        const PlaneRef wb = wedge_b_ref(mv, items[gd.first + q0 + h], k1, false);
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-          const TileElem<NP, OR> t(tid, m, c0, n3);
-          a[m] = fma2(buf[h][t.pp * PS + t.kp], Gk[t.off] * wb.at(t.k2, t.k3), a[m]);
-        }
-      }
-      __syncthreads();
+    for (int h = 0; h < NQ; ++h) {
+      lv[h] = s0 + h < ns && ((live[(size_t)slist[s0 + h].x * h1 + k1] >> tile) & 1u);
+      any = any || lv[h];
     }
+    if (!any) continue;
+    const bool act = lv[quarter];
+    float2 v[R1];
+    if (act) {
+      const float2 *src = slabs + (size_t)slist[s0 + quarter].x * H + (size_t)k1 * nt + c0;
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = src[(unsigned)((j + R2 * r) * rs + pq)];
+    }
+    float2 *pb = buf[quarter] + pq * PS;
+    if (act) {
+      RegFFT<NP, -1>::phase1(v, j, tw);
+#pragma unroll
+      for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
+      RegFFT<NP, -1>::phase2(v);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+        for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < NQ; ++h) {
+      if (!lv[h]) continue;
+      const int4 sl = slist[s0 + h];
+      const float *Gk = ns_group_table(mv, sl.y, false) + kb;
+      const PlaneRef wb = wedge_b_ref(mv, items[sl.z], k1, false);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const TE t(tid, m, c0, n3);
+        a[m] = fma2(buf[h][t.pp * PS + t.kp], Gk[t.off] * wb.at(t.k2, t.k3), a[m]);
+      }
+    }
+    __syncthreads();
   }
   if (OR == AX2) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const TileElem<NP, OR> t(tid, m, c0, n3);
+      const TE t(tid, m, c0, n3);
       acc[kb + t.off] = a[m];
     }
     return;
   }
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const TileElem<NP, OR> t(tid, m, c0, n3);
+    const TE t(tid, m, c0, n3);
     buf[0][t.pp * PS + t.kp] = a[m];
   }
   __syncthreads();
   float2 v[R1];
-  tile_fft<NP, 1>(buf[0] + pq * PS, tw, j, half == 0, v);
-  if (half == 0) {
+  tile_fft<NP, 1>(buf[0] + pq * PS, tw, j, quarter == 0, v);
+  if (quarter == 0) {
     float2 *dst = V + (size_t)k1 * n2 + c0;
     const int vs = h1 * n2;
 #pragma unroll
@@ -409,7 +415,7 @@ __global__ __launch_bounds__(pencil_threads<NP>()) void kP_fwd(const float2 *__r
   __shared__ float2 tw[NP];
   const int tid = threadIdx.x, ntiles = n2 / TP;
   const int k1 = blockIdx.x / ntiles, c0 = (blockIdx.x % ntiles) * TP;
-  for (int m = tid; m < NP; m += NTH) tw[m] = twg[m];
+  RegFFT<NP, 1>::fill_twiddles(tw, twg, tid, NTH);
   __syncthreads();
   const int pq = tid % TP, j = tid / TP;
   const float2 *src = T + (size_t)k1 * n2 + c0;
@@ -496,8 +502,8 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     u[r] = Ub[o];
     wa[r] = wab[wunit ? 0u : o];
   }
-  for (int m = tid; m < N1; m += NTH) tw1[m] = tw1g[m];
-  for (int m = tid; m < NC; m += NTH) twc[m] = twcg[m];
+  RegFFT<N1, 1>::fill_twiddles(tw1, tw1g, tid, NTH);
+  RegFFT<NC, 1>::fill_twiddles(twc, twcg, tid, NTH);
   for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
   __syncthreads();
   const uint32_t lrow = onslot ? rl[slot] : 0u;
@@ -647,8 +653,8 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
   const int tid = threadIdx.x, ip = blockIdx.x;
   const PlaneJob jb = jobs[blockIdx.y];
   const bool masked = jb.group >= 0;
-  for (int m = tid; m < N1; m += NTH) tw1[m] = tw1g[m];
-  for (int m = tid; m < NC; m += NTH) twc[m] = twcg[m];
+  RegFFT<N1, 1>::fill_twiddles(tw1, tw1g, tid, NTH);
+  RegFFT<NC, 1>::fill_twiddles(twc, twcg, tid, NTH);
   for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
   __syncthreads();
   const int slot = tid / RC2, j = tid % RC2;
@@ -818,12 +824,12 @@ template <int NP, int OR> struct PencilL {
                        c.n3, c.mv, c.items, c.groups, c.live, units, tw);
     SH_LAUNCHED();
   }
-  static void syn(const FusedCtx &c, int, const float2 *slabs, const int32_t *glist, int ng, float2 *acc,
+  static void syn(const FusedCtx &c, int, const float2 *slabs, const int4 *slist, int ns, float2 *acc,
                   int has_acc, float2 *V) {
     const int nt = OR == AX3 ? c.n2 : c.n3;
     const float2 *tw = OR == AX3 ? c.tw3 : c.tw2;
-    hipLaunchKernelGGL((kP_syn<NP, OR>), dim3(c.h1 * (nt / tp_of<NP>())), dim3(pblock_threads<NP>()), 0, c.stream,
-                       slabs, c.H, c.n2, c.n3, c.mv, c.items, c.groups, c.live, glist, ng, acc, has_acc, V, tw);
+    hipLaunchKernelGGL((kP_syn<NP, OR>), dim3(c.h1 * (nt / tp_of<NP>())), dim3(syn_threads<NP>()), 0, c.stream,
+                       slabs, c.H, c.n2, c.n3, c.mv, c.items, c.live, slist, ns, acc, has_acc, V, tw);
     SH_LAUNCHED();
   }
 };

@@ -68,10 +68,11 @@ struct FusedOps {
   // Synthesis plane pass: subbands (or the volume) -> slabs.
   void (*q_syn[2])(const FusedCtx &, int orient, const float *in, int64_t substride, int64_t is, float2 *slabs,
                    const PlaneJob *jobs, int njobs) = {nullptr, nullptr};
-  // Synthesis pencil pass over all tiles of one orientation for the groups
-  // glist[0..ng): AX2 writes its sum into acc; AX3 adds acc (has_acc), closes
-  // the sum and writes its inverse along k3 into V ([i3][k1][k2]).
-  void (*p_syn[2])(const FusedCtx &, int orient, const float2 *slabs, const int32_t *glist, int ng, float2 *acc,
+  // Synthesis pencil pass over all tiles of one orientation for the slabs
+  // slist[0..ns) = {slab, group, item of b-index q, 0}, summed in list order:
+  // AX2 writes its sum into acc; AX3 adds acc (has_acc), closes the sum and
+  // writes its inverse along k3 into V ([i3][k1][k2]).
+  void (*p_syn[2])(const FusedCtx &, int orient, const float2 *slabs, const int4 *slist, int ns, float2 *acc,
                    int has_acc, float2 *V) = {nullptr, nullptr};
   // X = FFT along k3 of T ([i3][k1][k2] -> [k1][k2][k3]).
   void (*p_fwd)(const FusedCtx &, const float2 *T, float2 *X) = nullptr;
