@@ -29,10 +29,12 @@ sys.path.insert(0, os.path.join(ROOT, "surfacelets.jl_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the plane kernels, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE runs (profiles/), scaled by a copy-kernel calibration;
-# None until measured.
-TRAFFIC = {}
+# HBM bytes per launch (read + written, averaged over the class's launches
+# in one step) of the plane kernels, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes over this bench, each scaled by what the counter reads for
+# a plain 256 MiB device copy (profiles/ns128_hbm_bytes.md).  Measured once for
+# the current kernels; not measured live.
+TRAFFIC = {"plane_forward": 233229088, "plane_inverse": 206284394}
 DIMS = (128, 128, 128)
 SEED = 20260
 
