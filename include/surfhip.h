@@ -97,6 +97,16 @@ int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p,
 int surfhip_ns_plan_destroy(surfhip_ns_plan *plan);
 int surfhip_ns_set_stream(surfhip_ns_plan *plan, void *stream);
 
+/* make_workspace(...; tune = true): the reference accepts `tune` and
+ * ignores it (src/transforms/ns_surfacelet.jl:90,147; FFTW's own planner
+ * flags play that role on the CPU).  Here it runs the plan's autotuner: for
+ * sizes outside the factored kernels' grids it times the plan's own launch
+ * sequence on its stream and keeps the fastest block shapes and radix orders
+ * (synchronises the stream).  config (NULL or 6 int32): factored (1/0),
+ * rows per axis-1 block, pencils per axis-2/3 block, and each axis's radix
+ * order as decimal digit pairs (e.g. 804 = radix 8 then 4). */
+int surfhip_ns_plan_tune(surfhip_ns_plan *plan, int32_t *config);
+
 /* analyze!: src/transforms/ns_surfacelet.jl:551-566.  x: n1 n2 n3 volume,
  * coeffs: (n1, n2, n3, S). */
 int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs);

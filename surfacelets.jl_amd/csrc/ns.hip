@@ -197,6 +197,7 @@ struct surfhip_ns_plan {
   int n1 = 0, n2 = 0, n3 = 0, h1 = 0;
   FftPlan1D f1{}, f2{}, f3{};
   int g_rows = 16, g_axis = 16;
+  float tuned_ms = 0.0f;            // timed sequence of the chosen configuration (0: not tuned)
   DevBuf<float> gana, gsyn, planes;
   DevBuf<float2> tw, X[2], acc[2], work;
   DevBuf<NSItem> ditems;
@@ -482,6 +483,71 @@ struct surfhip_ns_plan {
     SH_LAUNCHED();
   }
 
+  // Autotuner of the general path: block shapes (g_rows rows per axis-1
+  // block, g_axis pencils per axis-2/3 block) and the radix order of each
+  // axis's Stockham plan, chosen by timing this plan's own launch sequence
+  // for one subband (forward spectrum, one masked analysis, one masked
+  // accumulation: every kernel of a step) on the plan's stream.  Coordinate
+  // descent: each knob in turn over its candidates, the others fixed.
+  float time_config(const float *x, float *band) {
+    hipEvent_t a, b;
+    SH_HIP(hipEventCreate(&a));
+    SH_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      SH_HIP(hipEventRecord(a, stream));
+      forward_spectrum(x);
+      subbands_generic(1, 1, band);
+      accumulate(1, band);
+      SH_HIP(hipEventRecord(b, stream));
+      SH_HIP(hipEventSynchronize(b));
+      float ms = 0.0f;
+      SH_HIP(hipEventElapsedTime(&ms, a, b));
+      best = std::min(best, ms);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return best;
+  }
+  static void reverse_radices(FftPlan1D &pl) { std::reverse(pl.radix, pl.radix + pl.npass); }
+  void tune() {
+    if (fused) return;
+    DevBuf<float> x, band;
+    x.alloc((size_t)N * nch);
+    band.alloc((size_t)N * nch);
+    SH_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * (size_t)N * nch, stream));
+    float best = time_config(x.p, band.p);
+    auto try_knob = [&](int *knob, std::initializer_list<int> cands, auto fits) {
+      const int keep = *knob;
+      int pick = keep;
+      for (int v : cands) {
+        if (v == keep) continue;
+        *knob = v;
+        if (!fits()) continue;
+        const float ms = time_config(x.p, band.p);
+        if (ms < best) {
+          best = ms;
+          pick = v;
+        }
+      }
+      *knob = pick;
+    };
+    try_knob(&g_axis, {1, 2, 4, 8, 16, 32},
+             [&] { return lds_axis(n2) <= 65536 && lds_axis(n3) <= 65536; });
+    try_knob(&g_rows, {2, 4, 8, 16, 32},
+             [&] { return lds_rows_r2c() <= 65536 && lds_rows_c2r() <= 65536; });
+    for (FftPlan1D *pl : {&f1, &f2, &f3}) {
+      if (pl->npass < 2) continue;
+      reverse_radices(*pl);
+      const float ms = time_config(x.p, band.p);
+      if (ms < best)
+        best = ms;
+      else
+        reverse_radices(*pl);
+    }
+    tuned_ms = best;
+  }
+
   // X[ch] = rfft(channel ch of x)
   void forward_spectrum(const float *x) {
     static const NSItem none{};
@@ -722,6 +788,24 @@ int surfhip_ns_plan_destroy(surfhip_ns_plan *plan) {
     if (!plan) return;
     (void)hipStreamSynchronize(plan->stream);
     delete plan;
+  });
+}
+
+int surfhip_ns_plan_tune(surfhip_ns_plan *plan, int32_t *config) {
+  return guarded([&] {
+    if (!plan) fail(SURFHIP_ERR_ARG, "null plan");
+    plan->tune();
+    if (config) {
+      config[0] = plan->fused ? 1 : 0;
+      config[1] = plan->g_rows;
+      config[2] = plan->g_axis;
+      const FftPlan1D *pl[3] = {&plan->f1, &plan->f2, &plan->f3};
+      for (int a = 0; a < 3; ++a) {
+        int32_t code = 0;
+        for (int k = 0; k < pl[a]->npass; ++k) code = code * 100 + pl[a]->radix[k];
+        config[3 + a] = code;
+      }
+    }
   });
 }
 

@@ -165,6 +165,22 @@ class NSSurfaceletWorkspace:
     def bind_stream(self):
         _lib.call("surfhip_ns_set_stream", self._plan, ctypes_void(torch.cuda.current_stream(self.device).cuda_stream))
 
+    def tune(self):
+        """Run the plan's autotuner (block shapes and radix orders of the
+        general-size kernels; the factored kernels have nothing to tune) and
+        return the configuration it keeps."""
+        cfg = (ctypes.c_int32 * 6)()
+        self.bind_stream()
+        _lib.call("surfhip_ns_plan_tune", self._plan, cfg)
+        def radices(code):
+            out = []
+            while code:
+                out.append(code % 100)
+                code //= 100
+            return out[::-1]
+        return {"factored": bool(cfg[0]), "rows_per_block": int(cfg[1]), "pencils_per_block": int(cfg[2]),
+                "radix_orders": [radices(int(cfg[3 + a])) for a in range(3)]}
+
     def set_timing(self, enable):
         """Bracket every kernel launch with HIP events (a benchmarking hook)."""
         _lib.call("surfhip_ns_set_timing", self._plan, 1 if enable else 0)
@@ -204,8 +220,9 @@ def _eltype(x):
 def make_workspace(t, x_or_type, dims=None, *, flags=None, tasks=None, tune=False, prewarm=True,
                    cache_bandpass=True, device=None):
     """make_workspace(t, T, dims) or make_workspace(t, x): src/transforms/
-    ns_surfacelet.jl:88-173.  `flags`, `tune` and `cache_bandpass` are accepted
-    (the device evaluates the bandpass on the fly); `tasks` must be ≥ 1."""
+    ns_surfacelet.jl:88-173.  `flags` and `cache_bandpass` are accepted (the
+    device evaluates the bandpass on the fly); `tasks` must be ≥ 1.  `tune`
+    runs the plan's autotuner (the reference accepts and ignores it)."""
     if tasks is not None and tasks < 1:
         raise ArgumentError(f"tasks must be ≥ 1, got {tasks}")
     if isinstance(x_or_type, torch.Tensor):
@@ -218,6 +235,8 @@ def make_workspace(t, x_or_type, dims=None, *, flags=None, tasks=None, tune=Fals
         dt = _eltype(x_or_type)
         device = device or torch.device("cuda", torch.cuda.current_device())
     ws = NSSurfaceletWorkspace(t, dt, dims, device)
+    if tune:
+        ws.tune()
     if prewarm:
         z = torch.zeros(dims, dtype=dt, device=device)
         synthesize(ws, analyze(ws, as_colmajor(z)))
