@@ -71,6 +71,17 @@ typedef struct {
   int32_t diamond_power;
 } surfhip_s_params;
 
+/* The directional filter bank alone (make_hourglass / make_ndfb:
+ * src/directional/hourglass.jl:149-170, src/directional/ndfb.jl:91-104);
+ * levels = 0 is the hourglass bank. */
+typedef struct {
+  int32_t levels;
+  int32_t bernstein_order;
+  int32_t halfband_order;
+  double transition;
+  int32_t tight;
+} surfhip_ndfb_params;
+
 typedef struct surfhip_ns_plan surfhip_ns_plan;
 typedef struct surfhip_s_plan surfhip_s_plan;
 
@@ -145,6 +156,25 @@ int surfhip_ns_stream_finish(surfhip_ns_plan *plan, void *x);
  * ndfb_mask (src/directional/hourglass.jl:264-270). */
 int surfhip_ns_mask_host(const surfhip_ns_params *p, int32_t n1, int32_t n2,
                          int32_t n3, int32_t i, float *out);
+
+/* make_ndfb(T, dims; levels, ...) / make_hourglass(T, dims; ...)
+ * (src/directional/ndfb.jl:91-104, src/directional/hourglass.jl:149-170).
+ * The bank runs on the NS kernels as a plan of one scale without the
+ * pyramid: B = 1 and a zero lowpass, so subband 1 of the plan is zero and
+ * subbands 2..S are the bank's 3 * 2^levels directional subbands in
+ * ndfb_subband_index order (src/directional/ndfb.jl:184-190).
+ * surfhip_ns_analyze / _synthesize / streaming / _adjoint serve it:
+ * ndfb_decompose (:263-346) is the analysis, ndfb_reconstruct (:384-468) the
+ * synthesis (a plain sum when not tight, since the masks are then a partition
+ * of unity and the synthesis masks are 1). */
+int surfhip_ndfb_plan_create(surfhip_ns_plan **out, const surfhip_ndfb_params *p,
+                             int32_t n1, int32_t n2, int32_t n3, int32_t dtype,
+                             void *stream);
+/* ndfb_mask(fb, idx) (src/directional/ndfb.jl:215-220): the analysis mask of
+ * directional subband idx (1-based) on the rfft grid, as the device evaluates
+ * it; out has (n1/2+1) n2 n3 floats, axis 1 fastest.  Host only. */
+int surfhip_ndfb_mask_host(const surfhip_ndfb_params *p, int32_t n1, int32_t n2,
+                           int32_t n3, int32_t idx, float *out);
 
 /* Per-kernel timing (no counterpart in the reference: a profiling hook for
  * benchmarks).  With timing on, every kernel launch of the plan is bracketed

@@ -730,14 +730,17 @@ int surfhip_ns_nsubbands(const surfhip_ns_params *p, int32_t *out) {
   });
 }
 
-int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, int32_t n2, int32_t n3,
-                           int32_t dtype, void *stream) {
-  return guarded([&] {
+}  // extern "C"
+
+namespace {
+
+void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, int32_t n2, int32_t n3,
+                 int32_t dtype, void *stream, bool pyramid) {
     if (!out || !p) fail(SURFHIP_ERR_ARG, "null argument");
     *out = nullptr;
     if (dtype != SURFHIP_F32 && dtype != SURFHIP_C64) fail(SURFHIP_ERR_ARG, fmt("unknown dtype %d", dtype));
     auto P = std::make_unique<surfhip_ns_plan>();
-    ns_build_tables(*p, n1, n2, n3, P->T);
+    ns_build_tables(*p, n1, n2, n3, P->T, pyramid);
     P->dtype = dtype;
     P->nch = dtype == SURFHIP_C64 ? 2 : 1;
     P->stream = (hipStream_t)stream;
@@ -780,6 +783,60 @@ int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, in
     std::vector<float>().swap(P->T.gsyn);
     SH_HIP(hipDeviceSynchronize());
     *out = P.release();
+}
+
+// One directional bank (src/directional/ndfb.jl:91-127) as the NS parameters
+// of a single scale without the pyramid.
+surfhip_ns_params ndfb_as_ns(const surfhip_ndfb_params *p) {
+  if (p->levels < 0) fail(SURFHIP_ERR_ARG, "levels must be ≥ 0");
+  if (!(p->transition >= 0)) fail(SURFHIP_ERR_ARG, "transition must be ≥ 0");
+  surfhip_ns_params q{};
+  q.scales = 1;
+  q.direction_levels[0] = p->levels;
+  q.bernstein_order = p->bernstein_order;
+  q.halfband_order = p->halfband_order;
+  q.transition = p->transition;
+  q.tight = p->tight;
+  return q;
+}
+
+void mask_to_host(const NSTables &T, int32_t i, float *out) {
+  if (i < 1 || i > T.S) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", i, T.S));
+  const NSMaskView v = T.view();
+  const NSItem &it = T.items[i - 1];
+  for (int k3 = 0; k3 < T.n3; ++k3)
+    for (int k2 = 0; k2 < T.n2; ++k2)
+      for (int k1 = 0; k1 < T.h1; ++k1)
+        out[(size_t)k1 + (size_t)T.h1 * ((size_t)k2 + (size_t)T.n2 * k3)] = ns_mask(v, it, k1, k2, k3, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, int32_t n2, int32_t n3,
+                           int32_t dtype, void *stream) {
+  return guarded([&] { create_plan(out, p, n1, n2, n3, dtype, stream, true); });
+}
+
+int surfhip_ndfb_plan_create(surfhip_ns_plan **out, const surfhip_ndfb_params *p, int32_t n1, int32_t n2,
+                             int32_t n3, int32_t dtype, void *stream) {
+  return guarded([&] {
+    if (!p) fail(SURFHIP_ERR_ARG, "null argument");
+    const surfhip_ns_params q = ndfb_as_ns(p);
+    create_plan(out, &q, n1, n2, n3, dtype, stream, false);
+  });
+}
+
+int surfhip_ndfb_mask_host(const surfhip_ndfb_params *p, int32_t n1, int32_t n2, int32_t n3, int32_t idx,
+                           float *out) {
+  return guarded([&] {
+    if (!p || !out) fail(SURFHIP_ERR_ARG, "null argument");
+    const surfhip_ns_params q = ndfb_as_ns(p);
+    NSTables T;
+    ns_build_tables(q, n1, n2, n3, T, false);
+    if (idx < 1 || idx > T.S - 1) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", idx, T.S - 1));
+    mask_to_host(T, idx + 1, out);
   });
 }
 
@@ -918,13 +975,7 @@ int surfhip_ns_mask_host(const surfhip_ns_params *p, int32_t n1, int32_t n2, int
     if (!p || !out) fail(SURFHIP_ERR_ARG, "null argument");
     NSTables T;
     ns_build_tables(*p, n1, n2, n3, T);
-    if (i < 1 || i > T.S) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", i, T.S));
-    const NSMaskView v = T.view();
-    const NSItem &it = T.items[i - 1];
-    for (int k3 = 0; k3 < n3; ++k3)
-      for (int k2 = 0; k2 < n2; ++k2)
-        for (int k1 = 0; k1 < T.h1; ++k1)
-          out[(size_t)k1 + (size_t)T.h1 * ((size_t)k2 + (size_t)n2 * k3)] = ns_mask(v, it, k1, k2, k3, true);
+    mask_to_host(T, i, out);
   });
 }
 

@@ -141,7 +141,7 @@ int32_t ns_check_params(const surfhip_ns_params &p) {
   return S;
 }
 
-void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTables &T) {
+void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTables &T, bool pyramid) {
   const int32_t S = ns_check_params(p);
   if (n1 < 1 || n2 < 1 || n3 < 1)
     fail(SURFHIP_ERR_ARG, fmt("volume size must be positive, got (%d, %d, %d)", n1, n2, n3));
@@ -167,7 +167,7 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
       }
     }
   }
-  for (int j = 0; j < J; ++j) {
+  for (int j = 0; j < J && pyramid; ++j) {
     bool empty = true;
     for (int d = 0, off = 0; d < 3 && empty; off += nb[d], ++d)
       for (int k = 0; k < nb[d]; ++k)
@@ -297,7 +297,10 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
   // 205-208,226-227,265-272): group g = 3 (j - 1) + (i - 1) has
   // B = a1 a2 a3 sqrt(max(1 - (l1 l2 l3)^2, 0)) at pyramid level J - j times
   // the hourglass weight W_i = (W1, W2, W3) = (f0(M1), f1(M1) f0(M2),
-  // f1(M1) f1(M2)); the lowpass (last) is a1 a2 a3 at level J.
+  // f1(M1) f1(M2)); the lowpass (last) is a1 a2 a3 at level J.  Without the
+  // pyramid (the directional filter bank alone: src/directional/ndfb.jl:
+  // 215-241, src/directional/hourglass.jl:400-420) B = 1 and the lowpass is
+  // zero.
   const int G = 3 * J;
   T.ngroups = G + 1;
   T.gana.assign((size_t)T.ngroups * V, 0.0f);
@@ -315,13 +318,13 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
       for (int k2 = 0; k2 < n2; ++k2)
         for (int k3 = 0; k3 < n3; ++k3) {
           const size_t q = ((size_t)k1 * n2 + k2) * n3 + k3;
-          const double a3 = ax[k1] * ax[h1 + k2] * ax[h1 + n2 + k3];
+          const double a3 = pyramid ? ax[k1] * ax[h1 + k2] * ax[h1 + n2 + k3] : (low ? 0.0 : 1.0);
           if (low) {
             ga[q] = (float)a3;
             if (gs) gs[q] = (float)a3;
             continue;
           }
-          const double L = lx[k1] * lx[h1 + k2] * lx[h1 + n2 + k3];
+          const double L = pyramid ? lx[k1] * lx[h1 + k2] * lx[h1 + n2 + k3] : 0.0;
           const double B = a3 * std::sqrt(std::max(1.0 - L * L, 0.0));
           const size_t q2 = (size_t)k2 * n3 + k3;
           const double W = i == 1 ? hg1[q] : hg1c[q] * (i == 2 ? hg2[q2] : hg2c[q2]);

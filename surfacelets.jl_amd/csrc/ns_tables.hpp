@@ -121,7 +121,9 @@ struct NSTables {
 int32_t ns_check_params(const surfhip_ns_params &p);
 
 // Build every table in FP64 and round once (src/transforms/ns_surfacelet.jl:98-123).
-void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTables &out);
+// pyramid = false builds the directional filter bank alone (one scale, B = 1,
+// a zero lowpass): src/directional/ndfb.jl:91-241, hourglass.jl:149-420.
+void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTables &out, bool pyramid = true);
 
 // FP64 helpers shared with the decimated tables.
 double halfband_split(double x, int m);

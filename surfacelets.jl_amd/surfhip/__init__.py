@@ -10,6 +10,10 @@ from ._lib import ArgumentError, DeviceError, DimensionMismatch, lib
 from ._arrays import as_colmajor, empty_colmajor, from_numpy, to_numpy
 from . import ns as _ns
 from .ns import NSSurfacelet, NSSurfaceletWorkspace
+from .directional import (hourglass_decompose, hourglass_decompose_, hourglass_mask, hourglass_reconstruct,
+                          hourglass_reconstruct_, hourglass_weight, make_hourglass, make_ndfb, ndfb_decompose,
+                          ndfb_decompose_, ndfb_mask, ndfb_nsubbands, ndfb_reconstruct, ndfb_reconstruct_,
+                          ndfb_sharing, ndfb_subband_index, ndfb_subband_triple, ndfb_weight)
 
 
 def _dispatch(name, t, *args, **kw):
