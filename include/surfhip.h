@@ -176,6 +176,16 @@ int surfhip_ndfb_plan_create(surfhip_ns_plan **out, const surfhip_ndfb_params *p
 int surfhip_ndfb_mask_host(const surfhip_ndfb_params *p, int32_t n1, int32_t n2,
                            int32_t n3, int32_t idx, float *out);
 
+/* Group-factor tables (no counterpart in the reference: an inspection hook
+ * for tests).  A plan builds its tables on the device in FP64, rounded once
+ * to FP32; surfhip_ns_group_table copies table g (0-based; the lowpass is the
+ * last) of the analysis (synthesis = 0) or synthesis masks to out, H floats
+ * ordered [k1][k2][k3] with k3 fastest.  surfhip_ns_group_table_host builds
+ * the same table on the host with the same code. */
+int surfhip_ns_group_table(surfhip_ns_plan *plan, int32_t g, int32_t synthesis, float *out);
+int surfhip_ns_group_table_host(const surfhip_ns_params *p, int32_t n1, int32_t n2,
+                                int32_t n3, int32_t g, int32_t synthesis, float *out);
+
 /* Per-kernel timing (no counterpart in the reference: a profiling hook for
  * benchmarks).  With timing on, every kernel launch of the plan is bracketed
  * by HIP events on the plan's stream.  surfhip_ns_timing synchronises the

@@ -174,6 +174,18 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, float2 *out, int 
   }
 }
 
+// The group-factor tables on the device, one voxel per thread, from the
+// small FP64 inputs the host prepared (the same ns_group_voxel the host build
+// runs, so the two agree bit for bit).
+__global__ __launch_bounds__(NT) void k_group_tables(NSGroupInputs in, float *__restrict__ ga,
+                                                     float *__restrict__ gs, int64_t V) {
+  for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < V; q += (int64_t)gridDim.x * NT) {
+    const int k3 = (int)(q % in.n3);
+    const int64_t r = q / in.n3;
+    ns_group_voxel(in, (int)(r / in.n2), (int)(r % in.n2), k3, ga, gs, (size_t)V);
+  }
+}
+
 __global__ void k_zero(float2 *__restrict__ p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = make_float2(0.0f, 0.0f);
@@ -302,7 +314,14 @@ struct surfhip_ns_plan {
     // of it has a nonzero group factor and b-plane, for the analysis or the
     // synthesis masks.  Exact zeros of the tables decide it, so skipping dead
     // tiles changes no result.
-    const NSMaskView hv = T.view();
+    // The group tables live on the device; the liveness scan reads a host copy.
+    const size_t VT = (size_t)H * T.ngroups;
+    std::vector<float> hga(VT), hgs(T.tight ? 0 : VT);
+    SH_HIP(hipMemcpy(hga.data(), gana.p, VT * sizeof(float), hipMemcpyDeviceToHost));
+    if (!T.tight) SH_HIP(hipMemcpy(hgs.data(), gsyn.p, VT * sizeof(float), hipMemcpyDeviceToHost));
+    NSMaskView hv = T.view();
+    hv.gana = hga.data();
+    hv.gsyn = T.tight ? hga.data() : hgs.data();
     const bool noprune = std::getenv("SURFHIP_NS_NO_PRUNE") != nullptr;
     live.assign((size_t)nslabs * h1, 0u);
     const size_t V = (size_t)H;
@@ -740,7 +759,7 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
     *out = nullptr;
     if (dtype != SURFHIP_F32 && dtype != SURFHIP_C64) fail(SURFHIP_ERR_ARG, fmt("unknown dtype %d", dtype));
     auto P = std::make_unique<surfhip_ns_plan>();
-    ns_build_tables(*p, n1, n2, n3, P->T, pyramid);
+    ns_build_tables(*p, n1, n2, n3, P->T, pyramid, /*device_groups=*/true);
     P->dtype = dtype;
     P->nch = dtype == SURFHIP_C64 ? 2 : 1;
     P->stream = (hipStream_t)stream;
@@ -764,12 +783,35 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       check_lds(P->lds_axis(n3), "the axis-3 transform");
     }
     P->tw.upload(pool);
-    P->gana.upload(P->T.gana);
-    if (!P->T.gsyn.empty()) P->gsyn.upload(P->T.gsyn);
+    {
+      // Group-factor tables built on the device in FP64 (rounded once).
+      const size_t V = (size_t)P->h1 * n2 * n3;
+      P->gana.alloc((size_t)P->T.ngroups * V);
+      if (!P->T.tight) P->gsyn.alloc((size_t)P->T.ngroups * V);
+      DevBuf<double> t2, b3, hg2, hg2c, acc, lp;
+      t2.upload(P->T.g_t2);
+      b3.upload(P->T.g_b3);
+      hg2.upload(P->T.g_hg2);
+      hg2c.upload(P->T.g_hg2c);
+      acc.upload(P->T.g_acc);
+      lp.upload(P->T.g_lp);
+      NSGroupInputs in = P->T.group_inputs();
+      in.t2 = t2.p;
+      in.b3 = b3.p;
+      in.hg2 = hg2.p;
+      in.hg2c = hg2c.p;
+      in.acc = acc.p;
+      in.lp = lp.p;
+      const int64_t blocks = std::min<int64_t>(((int64_t)V + NT - 1) / NT, 65536);
+      hipLaunchKernelGGL(k_group_tables, dim3((unsigned)blocks), dim3(NT), 0, P->stream, in, P->gana.p,
+                         P->T.tight ? nullptr : P->gsyn.p, (int64_t)V);
+      SH_LAUNCHED();
+      SH_HIP(hipStreamSynchronize(P->stream));
+    }
     if (P->T.planes.empty()) P->T.planes.push_back(0.0f);
     P->planes.upload(P->T.planes);
     P->ditems.upload(P->T.items);
-    P->dview = NSMaskView{P->gana.p, P->T.gsyn.empty() ? P->gana.p : P->gsyn.p, P->planes.p, P->h1, n2, n3,
+    P->dview = NSMaskView{P->gana.p, P->T.tight ? P->gana.p : P->gsyn.p, P->planes.p, P->h1, n2, n3,
                           P->T.ngroups, P->T.tight, 0, 0};
     for (int ch = 0; ch < P->nch; ++ch) P->X[ch].alloc(P->H);
     if (P->fused) {
@@ -778,9 +820,9 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       for (int ch = 0; ch < P->nch; ++ch) P->acc[ch].alloc(P->H);
       P->work.alloc(P->H);
     }
-    // The host copies of the group tables are no longer needed.
-    std::vector<float>().swap(P->T.gana);
-    std::vector<float>().swap(P->T.gsyn);
+    // The group-table inputs are no longer needed on the host.
+    std::vector<double>().swap(P->T.g_t2);
+    std::vector<double>().swap(P->T.g_b3);
     SH_HIP(hipDeviceSynchronize());
     *out = P.release();
 }
@@ -837,6 +879,28 @@ int surfhip_ndfb_mask_host(const surfhip_ndfb_params *p, int32_t n1, int32_t n2,
     ns_build_tables(q, n1, n2, n3, T, false);
     if (idx < 1 || idx > T.S - 1) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", idx, T.S - 1));
     mask_to_host(T, idx + 1, out);
+  });
+}
+
+int surfhip_ns_group_table(surfhip_ns_plan *plan, int32_t g, int32_t synthesis, float *out) {
+  return guarded([&] {
+    if (!plan || !out) fail(SURFHIP_ERR_ARG, "null argument");
+    if (g < 0 || g >= plan->T.ngroups) fail(SURFHIP_ERR_ARG, fmt("group %d outside 0:%d", g, plan->T.ngroups - 1));
+    const float *src = (synthesis && !plan->T.tight ? plan->gsyn.p : plan->gana.p) + (size_t)g * plan->H;
+    SH_HIP(hipMemcpy(out, src, sizeof(float) * (size_t)plan->H, hipMemcpyDeviceToHost));
+  });
+}
+
+int surfhip_ns_group_table_host(const surfhip_ns_params *p, int32_t n1, int32_t n2, int32_t n3, int32_t g,
+                                int32_t synthesis, float *out) {
+  return guarded([&] {
+    if (!p || !out) fail(SURFHIP_ERR_ARG, "null argument");
+    NSTables T;
+    ns_build_tables(*p, n1, n2, n3, T);
+    if (g < 0 || g >= T.ngroups) fail(SURFHIP_ERR_ARG, fmt("group %d outside 0:%d", g, T.ngroups - 1));
+    const std::vector<float> &src = synthesis && !T.tight ? T.gsyn : T.gana;
+    const size_t V = (size_t)T.h1 * n2 * n3;
+    std::copy(src.begin() + (ptrdiff_t)(g * V), src.begin() + (ptrdiff_t)((g + 1) * V), out);
   });
 }
 

@@ -98,16 +98,7 @@ void transverse_axes(int i, int &a, int &b) {
 
 }  // namespace
 
-double halfband_split(double x, int m) {
-  double u = (1.0 + x) / 2, v = (1.0 - x) / 2;
-  if (u == 0.0) return 0.0;
-  double s = 0.0, c = 1.0;
-  for (int k = 0; k <= m; ++k) {
-    s += c * std::pow(v, k);
-    c = c * (m + k + 1) / (k + 1);
-  }
-  return std::pow(u, m + 1) * s;
-}
+double halfband_split(double x, int m) { return halfband_split_hd(x, m); }
 
 void rcos_axis(int n, long long wnum, long long wden, long long tnum, long long tden,
                std::vector<double> &prof, int &P, int &Q) {
@@ -141,7 +132,8 @@ int32_t ns_check_params(const surfhip_ns_params &p) {
   return S;
 }
 
-void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTables &T, bool pyramid) {
+void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTables &T, bool pyramid,
+                     bool device_groups) {
   const int32_t S = ns_check_params(p);
   if (n1 < 1 || n2 < 1 || n3 < 1)
     fail(SURFHIP_ERR_ARG, fmt("volume size must be positive, got (%d, %d, %d)", n1, n2, n3));
@@ -205,24 +197,7 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
       }
     }
   const size_t V = (size_t)h1 * n2 * n3;
-  std::vector<double> hg1(V), hg1c(V), hg2((size_t)n2 * n3), hg2c((size_t)n2 * n3);
-  std::vector<double> col(h1);
-  for (int k3 = 0; k3 < n3; ++k3)
-    for (int k2 = 0; k2 < n2; ++k2) {
-      std::fill(col.begin(), col.end(), 0.0);
-      for (int j3 = 0; j3 < np1; ++j3) {
-        double w = b3[(size_t)j3 * n3 + k3];
-        const double *src = &t2[((size_t)j3 * n2 + k2) * h1];
-        for (int k1 = 0; k1 < h1; ++k1) col[k1] += w * src[k1];
-      }
-      for (int k1 = 0; k1 < h1; ++k1) {
-        double f0 = halfband_split(col[k1], poly);
-        double f1 = halfband_split(-col[k1], poly);   // = 1 - f0, without cancellation
-        size_t q = ((size_t)k1 * n2 + (size_t)k2) * n3 + k3;
-        hg1[q] = T.tight ? std::sqrt(std::max(f0, 0.0)) : f0;
-        hg1c[q] = T.tight ? std::sqrt(std::max(f1, 0.0)) : f1;
-      }
-    }
+  std::vector<double> hg2((size_t)n2 * n3), hg2c((size_t)n2 * n3);
   for (int k3 = 0; k3 < n3; ++k3)
     for (int k2 = 0; k2 < n2; ++k2) {
       double m = 0.0;
@@ -303,35 +278,29 @@ void ns_build_tables(const surfhip_ns_params &p, int n1, int n2, int n3, NSTable
   // zero.
   const int G = 3 * J;
   T.ngroups = G + 1;
+  T.np1 = np1;
+  T.poly = poly;
+  T.pyramid = pyramid ? 1 : 0;
+  T.lsum = Lsum;
+  T.g_t2 = std::move(t2);
+  T.g_b3 = std::move(b3);
+  T.g_hg2 = std::move(hg2);
+  T.g_hg2c = std::move(hg2c);
+  T.g_acc.clear();
+  for (int lev = 0; lev <= J; ++lev) T.g_acc.insert(T.g_acc.end(), acc[lev].begin(), acc[lev].end());
+  T.g_lp.clear();
+  for (int lev = 0; lev < J; ++lev) T.g_lp.insert(T.g_lp.end(), lp[lev].begin(), lp[lev].end());
+  if (T.g_lp.empty()) T.g_lp.push_back(0.0);
+  T.gana.clear();
+  T.gsyn.clear();
+  if (device_groups) return;
   T.gana.assign((size_t)T.ngroups * V, 0.0f);
   if (!T.tight) T.gsyn.assign((size_t)T.ngroups * V, 0.0f);
-  else T.gsyn.clear();
-  for (int g = 0; g <= G; ++g) {
-    const bool low = g == G;
-    const int lev = low ? J : J - (g / 3 + 1);
-    const int i = g % 3 + 1;
-    const double *ax = acc[lev].data();
-    const double *lx = low ? nullptr : lp[lev].data();
-    float *ga = T.gana.data() + (size_t)g * V;
-    float *gs = T.tight ? nullptr : T.gsyn.data() + (size_t)g * V;
-    for (int k1 = 0; k1 < h1; ++k1)
-      for (int k2 = 0; k2 < n2; ++k2)
-        for (int k3 = 0; k3 < n3; ++k3) {
-          const size_t q = ((size_t)k1 * n2 + k2) * n3 + k3;
-          const double a3 = pyramid ? ax[k1] * ax[h1 + k2] * ax[h1 + n2 + k3] : (low ? 0.0 : 1.0);
-          if (low) {
-            ga[q] = (float)a3;
-            if (gs) gs[q] = (float)a3;
-            continue;
-          }
-          const double L = pyramid ? lx[k1] * lx[h1 + k2] * lx[h1 + n2 + k3] : 0.0;
-          const double B = a3 * std::sqrt(std::max(1.0 - L * L, 0.0));
-          const size_t q2 = (size_t)k2 * n3 + k3;
-          const double W = i == 1 ? hg1[q] : hg1c[q] * (i == 2 ? hg2[q2] : hg2c[q2]);
-          ga[q] = (float)(B * W);
-          if (gs) gs[q] = (float)B;
-        }
-  }
+  const NSGroupInputs in = T.group_inputs();
+  for (int k1 = 0; k1 < h1; ++k1)
+    for (int k2 = 0; k2 < n2; ++k2)
+      for (int k3 = 0; k3 < n3; ++k3)
+        ns_group_voxel(in, k1, k2, k3, T.gana.data(), T.tight ? nullptr : T.gsyn.data(), V);
 }
 
 }  // namespace surfhip

@@ -73,6 +73,8 @@ _SIGS = {
     "surfhip_ns_mask_host": (ctypes.c_int, [ctypes.POINTER(NSParams), _I, _I, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "surfhip_ndfb_plan_create": (ctypes.c_int, [ctypes.POINTER(_P), _P, _I, _I, _I, _I, _P]),
     "surfhip_ndfb_mask_host": (ctypes.c_int, [_P, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_float)]),
+    "surfhip_ns_group_table": (ctypes.c_int, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
+    "surfhip_ns_group_table_host": (ctypes.c_int, [ctypes.POINTER(NSParams), _I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "surfhip_ns_set_timing": (ctypes.c_int, [_P, _I]),
     "surfhip_ns_timing": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                          ctypes.c_char_p, ctypes.c_size_t]),
