@@ -34,9 +34,12 @@ constexpr int NT = 256;
 // Real rows of length n1 (row r at x[(r n1 + t) xs]) -> half-spectrum rows
 // out[r h1 + k].  Two rows share one complex FFT: z = a + i b, then
 // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i.
-__global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, int64_t xs,
-                                                 float2 *__restrict__ out, int n1, int h1, int nrows,
-                                                 int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+__global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, int64_t xs, int64_t x_bs,
+                                                 float2 *__restrict__ out, int64_t out_bs, int n1, int h1,
+                                                 int nrows, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+  // Batch entry blockIdx.y: its own input and output.
+  x += blockIdx.y * x_bs;
+  out += blockIdx.y * out_bs;
   extern __shared__ float2 smem[];
   const int P = G / 2;
   float2 *tw = smem;
@@ -76,9 +79,12 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, in
 // Pairs of rows form z = a + i b through their Hermitian extensions; the
 // imaginary parts of the DC and (even n1) Nyquist bins are ignored, as a c2r
 // transform does.
-__global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, float *__restrict__ out,
-                                                 int64_t os, float scale, int n1, int h1, int nrows, int G,
-                                                 FftPlan1D pl, const float2 *__restrict__ twg) {
+__global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, int64_t in_bs,
+                                                 float *__restrict__ out, int64_t os, int64_t out_bs, float scale,
+                                                 int n1, int h1, int nrows, int G, FftPlan1D pl,
+                                                 const float2 *__restrict__ twg) {
+  in += blockIdx.y * in_bs;
+  out += blockIdx.y * out_bs;
   extern __shared__ float2 smem[];
   const int P = G / 2;
   float2 *tw = smem;
@@ -124,11 +130,21 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
 
 // Complex FFT along axis AXIS (2 or 3) of an rfft-grid array, G pencils per
 // block that are consecutive in k1.  MODE 0: out = FFT(in); MODE 1:
-// out = FFT(in . mask) (analysis); MODE 2: out += mask . FFT(in) (synthesis).
+// out = FFT(in . mask) (analysis); MODE 2: out += mask . FFT(in)
+// (synthesis).  Batch entry b = blockIdx.y reads in + b in_bs, writes
+// out + b out_bs and uses items[b].  MODE 3 (synthesis of a batch into one
+// sum): grid.y = 1 and the block runs entries 0..nb-1 in order, adding
+// mask_b . FFT(in_b) to out exactly as nb MODE 2 launches would.
+constexpr int AXIS_MAXE = 16;  // elements per thread: n G <= 4096 (the LDS limit)
 template <int AXIS, int DIR, int MODE>
-__global__ __launch_bounds__(NT) void k_axis(const float2 *in, float2 *out, int h1,
-                                             int n2, int n3, int G, FftPlan1D pl,
-                                             const float2 *__restrict__ twg, NSMaskView mv, NSItem it) {
+__global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, float2 *out, int64_t out_bs,
+                                             int h1, int n2, int n3, int G, FftPlan1D pl,
+                                             const float2 *__restrict__ twg, NSMaskView mv,
+                                             const NSItem *__restrict__ items, int nb) {
+  if (MODE != 3) {
+    in += blockIdx.y * in_bs;
+    out += blockIdx.y * out_bs;
+  }
   extern __shared__ float2 smem[];
   const int n = AXIS == 2 ? n2 : n3;
   float2 *tw = smem;
@@ -142,34 +158,56 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, float2 *out, int 
   const int u0 = ub * G;
   const int64_t sa = AXIS == 2 ? (int64_t)h1 : (int64_t)h1 * n2;
   const int64_t base = AXIS == 2 ? (int64_t)v * h1 * n2 : (int64_t)v * h1;
-  for (int idx = tid; idx < n * G; idx += NT) {
-    const int t = idx / G, g = idx - t * G;
-    const int k1 = u0 + g;
-    float2 val = make_float2(0.0f, 0.0f);
-    if (k1 < h1) {
-      val = in[base + k1 + t * sa];
-      if (MODE == 1) {
+  const int nent = MODE == 3 ? nb : 1;
+  float2 sum[AXIS_MAXE];
+  for (int e = 0; e < nent; ++e) {
+    const NSItem it = items[MODE == 3 ? e : blockIdx.y];
+    const float2 *src = in + (MODE == 3 ? e * in_bs : 0);
+    for (int idx = tid; idx < n * G; idx += NT) {
+      const int t = idx / G, g = idx - t * G;
+      const int k1 = u0 + g;
+      float2 val = make_float2(0.0f, 0.0f);
+      if (k1 < h1) {
+        val = src[base + k1 + t * sa];
+        if (MODE == 1) {
+          const int k2 = AXIS == 2 ? t : v, k3 = AXIS == 2 ? v : t;
+          val = cscale(val, ns_mask(mv, it, k1, k2, k3, true));
+        }
+      }
+      a[idx] = val;
+    }
+    __syncthreads();
+    float2 *res = lds_fft<DIR>(a, b, pl, G, tw, tid, NT);
+#pragma unroll
+    for (int m = 0; m < AXIS_MAXE; ++m) {
+      const int idx = tid + m * NT;
+      if (idx >= n * G) break;
+      const int t = idx / G, g = idx - t * G;
+      const int k1 = u0 + g;
+      if (k1 >= h1) continue;
+      const int64_t o = base + k1 + t * sa;
+      const float2 r = res[idx];
+      if (MODE == 2 || MODE == 3) {
         const int k2 = AXIS == 2 ? t : v, k3 = AXIS == 2 ? v : t;
-        val = cscale(val, ns_mask(mv, it, k1, k2, k3, true));
+        const float mk = ns_mask(mv, it, k1, k2, k3, false);
+        const float2 acc = MODE == 3 && e > 0 ? sum[m] : out[o];
+        const float2 nv = make_float2(acc.x + mk * r.x, acc.y + mk * r.y);
+        if (MODE == 2) out[o] = nv;
+        else sum[m] = nv;
+      } else {
+        out[o] = r;
       }
     }
-    a[idx] = val;
+    __syncthreads();  // a and b are refilled by the next entry
   }
-  __syncthreads();
-  float2 *res = lds_fft<DIR>(a, b, pl, G, tw, tid, NT);
-  for (int idx = tid; idx < n * G; idx += NT) {
-    const int t = idx / G, g = idx - t * G;
-    const int k1 = u0 + g;
-    if (k1 >= h1) continue;
-    const int64_t o = base + k1 + t * sa;
-    float2 r = res[idx];
-    if (MODE == 2) {
-      const int k2 = AXIS == 2 ? t : v, k3 = AXIS == 2 ? v : t;
-      const float m = ns_mask(mv, it, k1, k2, k3, false);
-      float2 acc = out[o];
-      out[o] = make_float2(acc.x + m * r.x, acc.y + m * r.y);
-    } else {
-      out[o] = r;
+  if (MODE == 3) {
+#pragma unroll
+    for (int m = 0; m < AXIS_MAXE; ++m) {
+      const int idx = tid + m * NT;
+      if (idx >= n * G) break;
+      const int t = idx / G, g = idx - t * G;
+      const int k1 = u0 + g;
+      if (k1 < h1) out[base + k1 + t * sa] = sum[m];
     }
   }
 }
@@ -209,6 +247,7 @@ struct surfhip_ns_plan {
   int n1 = 0, n2 = 0, n3 = 0, h1 = 0;
   FftPlan1D f1{}, f2{}, f3{};
   int g_rows = 16, g_axis = 16;
+  int nwork = 1;                    // subbands per general-path launch (work holds nwork spectra)
   float tuned_ms = 0.0f;            // timed sequence of the chosen configuration (0: not tuned)
   DevBuf<float> gana, gsyn, planes;
   DevBuf<float2> tw, X[2], acc[2], work;
@@ -477,24 +516,30 @@ struct surfhip_ns_plan {
 
   int rows() const { return n2 * n3; }
 
-  void r2c(const float *x, int64_t xs, float2 *out) {
+  // General-path launchers; nb > 1 runs a batch of nb subbands (grid.y),
+  // entry b at in + b in_bs / out + b out_bs with item item0 + b.
+  void r2c(const float *x, int64_t xs, float2 *out, int nb = 1, int64_t x_bs = 0, int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
-    hipLaunchKernelGGL(k_rows_r2c, dim3(grid), dim3(NT), lds_rows_r2c(), stream, x, xs, out, n1, h1, rows(),
-                       g_rows, f1, tw.p);
+    hipLaunchKernelGGL(k_rows_r2c, dim3(grid, nb), dim3(NT), lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1,
+                       h1, rows(), g_rows, f1, tw.p);
     SH_LAUNCHED();
   }
-  void c2r(const float2 *in, float *out, int64_t os, float scale) {
+  void c2r(const float2 *in, float *out, int64_t os, float scale, int nb = 1, int64_t in_bs = 0,
+           int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
-    hipLaunchKernelGGL(k_rows_c2r, dim3(grid), dim3(NT), lds_rows_c2r(), stream, in, out, os, scale, n1, h1,
-                       rows(), g_rows, f1, tw.p);
+    hipLaunchKernelGGL(k_rows_c2r, dim3(grid, nb), dim3(NT), lds_rows_c2r(), stream, in, in_bs, out, os, out_bs,
+                       scale, n1, h1, rows(), g_rows, f1, tw.p);
     SH_LAUNCHED();
   }
-  template <int AXIS, int DIR, int MODE> void axis(const float2 *in, float2 *out, const NSItem &it) {
+  template <int AXIS, int DIR, int MODE>
+  void axis(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
     const int ublocks = (h1 + g_axis - 1) / g_axis;
     const int grid = ublocks * (AXIS == 2 ? n3 : n2);
     const FftPlan1D &pl = AXIS == 2 ? f2 : f3;
-    hipLaunchKernelGGL((k_axis<AXIS, DIR, MODE>), dim3(grid), dim3(NT), lds_axis(pl.n), stream, in, out, h1, n2,
-                       n3, g_axis, pl, tw.p, dview, it);
+    if ((int64_t)pl.n * g_axis > (int64_t)AXIS_MAXE * NT)
+      fail(SURFHIP_ERR_ARG, fmt("axis block of %d x %d elements exceeds %d", pl.n, g_axis, AXIS_MAXE * NT));
+    hipLaunchKernelGGL((k_axis<AXIS, DIR, MODE>), dim3(grid, MODE == 3 ? 1 : nb), dim3(NT), lds_axis(pl.n), stream,
+                       in, in_bs, out, out_bs, h1, n2, n3, g_axis, pl, tw.p, dview, ditems.p + item0, nb);
     SH_LAUNCHED();
   }
   void zero(float2 *p, int64_t n) {
@@ -505,9 +550,10 @@ struct surfhip_ns_plan {
   // Autotuner of the general path: block shapes (g_rows rows per axis-1
   // block, g_axis pencils per axis-2/3 block) and the radix order of each
   // axis's Stockham plan, chosen by timing this plan's own launch sequence
-  // for one subband (forward spectrum, one masked analysis, one masked
-  // accumulation: every kernel of a step) on the plan's stream.  Coordinate
-  // descent: each knob in turn over its candidates, the others fixed.
+  // for one batch of nwork subbands (forward spectrum, a batched masked
+  // analysis, a batched masked accumulation: every kernel of a step, at the
+  // batch size a step launches) on the plan's stream.  Coordinate descent:
+  // each knob in turn over its candidates, the others fixed.
   float time_config(const float *x, float *band) {
     hipEvent_t a, b;
     SH_HIP(hipEventCreate(&a));
@@ -516,8 +562,9 @@ struct surfhip_ns_plan {
     for (int rep = 0; rep < 3; ++rep) {
       SH_HIP(hipEventRecord(a, stream));
       forward_spectrum(x);
-      subbands_generic(1, 1, band);
-      accumulate(1, band);
+      const int nw = std::min(nwork, T.S - 1);
+      subbands_generic(1, nw, band);
+      accumulate_generic(1, nw, band);
       SH_HIP(hipEventRecord(b, stream));
       SH_HIP(hipEventSynchronize(b));
       float ms = 0.0f;
@@ -533,7 +580,7 @@ struct surfhip_ns_plan {
     if (fused) return;
     DevBuf<float> x, band;
     x.alloc((size_t)N * nch);
-    band.alloc((size_t)N * nch);
+    band.alloc((size_t)N * nch * std::max(1, std::min(nwork, T.S - 1)));
     SH_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * (size_t)N * nch, stream));
     float best = time_config(x.p, band.p);
     auto try_knob = [&](int *knob, std::initializer_list<int> cands, auto fits) {
@@ -569,7 +616,6 @@ struct surfhip_ns_plan {
 
   // X[ch] = rfft(channel ch of x)
   void forward_spectrum(const float *x) {
-    static const NSItem none{};
     cached_group = -1;
     for (int ch = 0; ch < nch; ++ch) {
       if (fused) {
@@ -579,28 +625,40 @@ struct surfhip_ns_plan {
         continue;
       }
       r2c(x + ch, nch, X[ch].p);
-      axis<2, -1, 0>(X[ch].p, X[ch].p, none);
-      axis<3, -1, 0>(X[ch].p, X[ch].p, none);
+      axis<2, -1, 0>(X[ch].p, X[ch].p);
+      axis<3, -1, 0>(X[ch].p, X[ch].p);
     }
   }
-  // Generic path: subbands item0 .. item0+nw-1 into out (subband stride N nch).
+  // Generic path: subbands item0 .. item0+nw-1 into out (subband stride
+  // N nch), up to nwork of them per launch.
   void subbands_generic(int item0, int nw, float *out) {
-    static const NSItem none{};
     const float scale = 1.0f / (float)N;
     for (int ch = 0; ch < nch; ++ch)
-      for (int w = 0; w < nw; ++w) {
-        axis<3, 1, 1>(X[ch].p, work.p, T.items[item0 + w]);
-        axis<2, 1, 0>(work.p, work.p, none);
-        c2r(work.p, out + (int64_t)w * N * nch + ch, nch, scale);
+      for (int w0 = 0; w0 < nw; w0 += nwork) {
+        const int nb = std::min(nwork, nw - w0);
+        axis<3, 1, 1>(X[ch].p, work.p, item0 + w0, nb, 0, H);
+        axis<2, 1, 0>(work.p, work.p, 0, nb, H, H);
+        c2r(work.p, out + (int64_t)w0 * N * nch + ch, nch, scale, nb, H, N * nch);
+      }
+  }
+  // Generic path: acc += sum over subbands item0 .. item0+nw-1 (in order) of
+  // rfft(band) . M_i, bands at in (subband stride N nch).
+  void accumulate_generic(int item0, int nw, const float *in) {
+    for (int ch = 0; ch < nch; ++ch)
+      for (int w0 = 0; w0 < nw; w0 += nwork) {
+        const int nb = std::min(nwork, nw - w0);
+        r2c(in + (int64_t)w0 * N * nch + ch, nch, work.p, nb, N * nch, H);
+        axis<2, -1, 0>(work.p, work.p, 0, nb, H, H);
+        if (nb == 1)
+          axis<3, -1, 2>(work.p, acc[ch].p, item0 + w0);
+        else
+          axis<3, -1, 3>(work.p, acc[ch].p, item0 + w0, nb, H, 0);
       }
   }
   void analyze(const float *x, float *coeffs) {
     forward_spectrum(x);
     if (!fused) {
-      for (size_t g = 0; g < T.group_first.size(); ++g) {
-        const int i0 = T.group_first[g], nw = T.group_count[g];
-        subbands_generic(i0, nw, coeffs + (int64_t)i0 * N * nch);
-      }
+      subbands_generic(1, T.S - 1, coeffs + (int64_t)N * nch);
       subbands_generic(0, 1, coeffs);
       return;
     }
@@ -650,8 +708,8 @@ struct surfhip_ns_plan {
   void synthesize(const float *coeffs, float *x) {
     if (!fused) {
       inverse_begin();
-      for (int i = 1; i < T.S; ++i) accumulate(i, coeffs + (int64_t)i * N * nch);
-      accumulate(0, coeffs);
+      accumulate_generic(1, T.S - 1, coeffs + (int64_t)N * nch);
+      accumulate_generic(0, 1, coeffs);
       finish(x);
       return;
     }
@@ -671,7 +729,6 @@ struct surfhip_ns_plan {
   }
   // acc += rfft(band) . M_i
   void accumulate(int i, const float *band) {
-    static const NSItem none{};
     if (fused) {
       const FusedCtx c = ctx();
       const int g = group_of(i);
@@ -683,15 +740,9 @@ struct surfhip_ns_plan {
         timed(2, [&] { fops.q_syn[o](c, o, band + ch, 0, nch, R[ch].p, djobs.p + jb, 1); });
       return;
     }
-    const NSItem &it = T.items[i];
-    for (int ch = 0; ch < nch; ++ch) {
-      r2c(band + ch, nch, work.p);
-      axis<2, -1, 0>(work.p, work.p, none);
-      axis<3, -1, 2>(work.p, acc[ch].p, it);
-    }
+    accumulate_generic(i, 1, band);
   }
   void finish(float *x) {
-    static const NSItem none{};
     if (fused) {
       // Slabs that received no subband hold no partial sum: clear them.
       for (int u = 0; u < nslabs; ++u)
@@ -701,8 +752,8 @@ struct surfhip_ns_plan {
       return;
     }
     for (int ch = 0; ch < nch; ++ch) {
-      axis<3, 1, 0>(acc[ch].p, work.p, none);
-      axis<2, 1, 0>(work.p, work.p, none);
+      axis<3, 1, 0>(acc[ch].p, work.p);
+      axis<2, 1, 0>(work.p, work.p);
       c2r(work.p, x + ch, nch, 1.0f / (float)N);
     }
   }
@@ -818,7 +869,10 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       P->build_fused();
     } else {
       for (int ch = 0; ch < P->nch; ++ch) P->acc[ch].alloc(P->H);
-      P->work.alloc(P->H);
+      // Batched general-path launches: up to nwork subbands per launch, as
+      // long as their work buffers stay within 8 volumes of the spectrum.
+      P->nwork = 8;
+      P->work.alloc((size_t)P->H * P->nwork);
     }
     // The group-table inputs are no longer needed on the host.
     std::vector<double>().swap(P->T.g_t2);
