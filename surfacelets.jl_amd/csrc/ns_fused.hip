@@ -206,7 +206,9 @@ template <int NP, int OR, int NTH = pblock_threads<NP>()> struct TileElem {
 // Inverse (DIR = +1) or forward FFT of the TP pencils in buf (natural order)
 // by one half-block: thread (pq, j); the result stays in registers in phase-2
 // order, v[uu R2 + qq] = element (j ROWS + uu) + R1 qq of pencil pq.
-template <int NP, int DIR>
+// WAVE: the R2 threads of each pencil are lanes of one wave, so the
+// exchange needs only wave-level fences.
+template <int NP, int DIR, bool WAVE = false>
 __device__ __forceinline__ void tile_fft(float2 *pb, const float2 *tw, int j, bool active,
                                          float2 (&v)[FourStep<NP>::R1]) {
   constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2;
@@ -215,12 +217,18 @@ __device__ __forceinline__ void tile_fft(float2 *pb, const float2 *tw, int j, bo
     for (int r = 0; r < R1; ++r) v[r] = pb[j + R2 * r];
     RegFFT<NP, DIR>::phase1(v, j, tw);
   }
-  __syncthreads();
+  if (WAVE)
+    wave_sync();
+  else
+    __syncthreads();
   if (active) {
 #pragma unroll
     for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
   }
-  __syncthreads();
+  if (WAVE)
+    wave_sync();
+  else
+    __syncthreads();
   if (active) {
 #pragma unroll
     for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
@@ -258,7 +266,11 @@ __global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *
     const TileElem<NP, OR> t(tid, m, c0, n3);
     y[m] = cscale(Xk[t.off], Gk[t.off]);
   }
-  const int half = tid / HT, lt = tid % HT, pq = lt % TP, j = lt / TP;
+  // The R2 lanes of a pencil lie in one wave (see kP_syn).
+  constexpr int PL = 64 / R2;
+  static_assert(HT % 64 == 0 && TP % PL == 0, "pencils of a half fill whole waves");
+  const int half = tid / HT, lt = tid % HT;
+  const int j = (lt / PL) % R2, pq = lt % PL + PL * (lt / 64);
   const int rs = h1 * nt;
   for (int q0 = 0; q0 < gd.nB; q0 += 2) {
     bool lv[2];
@@ -277,7 +289,7 @@ __global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *
     __syncthreads();
     const bool act = lv[half];
     float2 v[R1];
-    tile_fft<NP, 1>(buf[half] + pq * PS, tw, j, act, v);
+    tile_fft<NP, 1, true>(buf[half] + pq * PS, tw, j, act, v);
     if (act) {
       float2 *dst = slabs + (size_t)(gd.slab0 + q0 + half) * H + (size_t)k1 * nt + c0;
 #pragma unroll
@@ -327,7 +339,14 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
     a[m] = (OR == AX3 && has_acc) ? acc[kb + t.off] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
-  const int quarter = tid / QT, lt = tid % QT, pq = lt % TP, j = lt / TP;
+  // Thread lt of a quarter: pencil pq = pq_lo + PL pq_hi, lane j; the R2
+  // lanes of a pencil lie in one wave (PL lanes apart), so its exchange needs
+  // only a wave-level fence, and PL consecutive lanes read PL consecutive
+  // pencils.
+  constexpr int PL = 64 / R2;
+  static_assert(QT % 64 == 0 && TP % PL == 0, "pencils of a quarter fill whole waves");
+  const int quarter = tid / QT, lt = tid % QT;
+  const int j = (lt / PL) % R2, pq = lt % PL + PL * (lt / 64);
   const int rs = h1 * nt;
   for (int s0 = 0; s0 < ns; s0 += NQ) {
     bool lv[NQ];
@@ -351,13 +370,13 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
 #pragma unroll
       for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
     }
-    __syncthreads();
+    wave_sync();  // the exchange region of a pencil is private to its R2 lanes
     if (act) {
 #pragma unroll
       for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
       RegFFT<NP, -1>::phase2(v);
     }
-    __syncthreads();
+    wave_sync();
     if (act) {
 #pragma unroll
       for (int uu = 0; uu < ROWS; ++uu)
@@ -394,7 +413,7 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
   }
   __syncthreads();
   float2 v[R1];
-  tile_fft<NP, 1>(buf[0] + pq * PS, tw, j, quarter == 0, v);
+  tile_fft<NP, 1, true>(buf[0] + pq * PS, tw, j, quarter == 0, v);
   if (quarter == 0) {
     float2 *dst = V + (size_t)k1 * n2 + c0;
     const int vs = h1 * n2;
