@@ -451,6 +451,38 @@ struct surfhip_s_plan {
   DecTables T;
   int dtype = SURFHIP_F32, nch = 1;
   hipStream_t stream = nullptr;
+  // The three hourglasses of a scale are independent until their spectra are
+  // summed: their passes run on one stream each, forked from and joined back
+  // to `stream` with events.  `ls` is the stream the launch helpers use.
+  hipStream_t ls = nullptr;
+  hipStream_t kst[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t kfork = nullptr, kjoin[3] = {nullptr, nullptr, nullptr};
+  ~surfhip_s_plan() {
+    for (int K = 0; K < 3; ++K) {
+      if (kst[K]) (void)hipStreamDestroy(kst[K]);
+      if (kjoin[K]) (void)hipEventDestroy(kjoin[K]);
+    }
+    if (kfork) (void)hipEventDestroy(kfork);
+  }
+  void make_lanes() {
+    for (int K = 0; K < 3; ++K) {
+      SH_HIP(hipStreamCreateWithFlags(&kst[K], hipStreamNonBlocking));
+      SH_HIP(hipEventCreateWithFlags(&kjoin[K], hipEventDisableTiming));
+    }
+    SH_HIP(hipEventCreateWithFlags(&kfork, hipEventDisableTiming));
+  }
+  // Later work of the hourglass streams follows everything already on `stream`.
+  void fork_k() {
+    SH_HIP(hipEventRecord(kfork, stream));
+    for (int K = 0; K < 3; ++K) SH_HIP(hipStreamWaitEvent(kst[K], kfork, 0));
+  }
+  // Later work of `stream` follows everything already on the hourglass streams.
+  void join_k() {
+    for (int K = 0; K < 3; ++K) {
+      SH_HIP(hipEventRecord(kjoin[K], kst[K]));
+      SH_HIP(hipStreamWaitEvent(stream, kjoin[K], 0));
+    }
+  }
   std::vector<std::unique_ptr<DecLevelDev>> lev;
   std::vector<std::unique_ptr<DecScaleDev>> sc;
   DevBuf<float> masks;
@@ -479,7 +511,7 @@ struct surfhip_s_plan {
       blocks = axis == 1 ? ub * (m[1] / seg) * m[2] : ub * m[1] * (m[2] / seg);
     }
 #define SH_C2C(AX, D) \
-  hipLaunchKernelGGL((k_c2c<AX, D>), dim3((unsigned)blocks), dim3(DT), lds, stream, A, A, m[0], m[1], m[2], seg, G, pl, tw.p)
+  hipLaunchKernelGGL((k_c2c<AX, D>), dim3((unsigned)blocks), dim3(DT), lds, ls, A, A, m[0], m[1], m[2], seg, G, pl, tw.p)
     if (axis == 0) { if (dir < 0) SH_C2C(1, -1); else SH_C2C(1, 1); }
     else if (axis == 1) { if (dir < 0) SH_C2C(2, -1); else SH_C2C(2, 1); }
     else { if (dir < 0) SH_C2C(3, -1); else SH_C2C(3, 1); }
@@ -490,7 +522,7 @@ struct surfhip_s_plan {
     for (int ax = 0; ax < 3; ++ax) launch_c2c(ax, dir, A, m, m[ax]);
   }
   void zero(float2 *p, int64_t n) {
-    hipLaunchKernelGGL(k_dzero, dim3(grid_for(n)), dim3(DT), 0, stream, p, n);
+    hipLaunchKernelGGL(k_dzero, dim3(grid_for(n)), dim3(DT), 0, ls, p, n);
     SH_LAUNCHED();
   }
   BlockMap block_map(int j, int K) const {
@@ -521,19 +553,20 @@ struct surfhip_s_plan {
     const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
     const unsigned blocks = (unsigned)((nlines + G - 1) / G);
     if (c2r)
-      hipLaunchKernelGGL(k_c2r_lines, dim3(blocks), dim3(DT), lds, stream, S, coef, stride, M, G, pl, tw.p);
+      hipLaunchKernelGGL(k_c2r_lines, dim3(blocks), dim3(DT), lds, ls, S, coef, stride, M, G, pl, tw.p);
     else
-      hipLaunchKernelGGL(k_r2c_lines, dim3(blocks), dim3(DT), lds, stream, (const float *)coef, stride, S, M, G, pl,
+      hipLaunchKernelGGL(k_r2c_lines, dim3(blocks), dim3(DT), lds, ls, (const float *)coef, stride, S, M, G, pl,
                          tw.p, dbl);
     SH_LAUNCHED();
   }
 
   void analyze_channel(const float *x, float *coef) {
+    ls = stream;
     const int64_t os = nch;
     Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
     float2 *X = spectra[0].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, stream, x, (int64_t)nch, X, N, 1.0f);
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, ls, x, (int64_t)nch, X, N, 1.0f);
     SH_LAUNCHED();
     fft3(-1, X, n);
     for (int j = 0; j < T.J; ++j) {
@@ -541,21 +574,23 @@ struct surfhip_s_plan {
       float2 *low = spectra[j + 1].p;
       zero(low, (int64_t)pl.nlow[0] * pl.nlow[1] * pl.nlow[2]);
       const PyrDev &pd = lev[j]->pd;
-      hipLaunchKernelGGL(k_pyr_split, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, stream,
+      hipLaunchKernelGGL(k_pyr_split, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls,
                          spectra[j].p, low, pd);
       SH_LAUNCHED();
       const DecScaleT &s = T.scales[j];
+      fork_k();
       for (int K = 0; K < 3; ++K) {
+        ls = kst[K];
         Dims3 hd = half_dims(s.n, K);
         Dims3 sb = subband_size(s, K);
         const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
         float2 *S = Sbuf[K].p;
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
-        hipLaunchKernelGGL(k_masked, dim3(grid_for(HN)), dim3(DT), 0, stream, spectra[j].p, s.n[0], s.n[1], S,
+        hipLaunchKernelGGL(k_masked, dim3(grid_for(HN)), dim3(DT), 0, ls, spectra[j].p, s.n[0], s.n[1], S,
                            masks.p + s.mask_off[K], hd[0], hd[1], hd[2], scale);
         SH_LAUNCHED();
         for (const IRCStageT &st : s.stages[K]) {
-          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, stream, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
                              filt.p + st.off[0], filt.p + st.off[1], 0);
           SH_LAUNCHED();
         }
@@ -565,22 +600,25 @@ struct surfhip_s_plan {
           if (d != K) launch_c2c(d, 1, S, hd, bsize[d]);
         lines(true, j, K, S, coef, os);
       }
+      ls = stream;
     }
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
     fft3(1, C, nl);
-    hipLaunchKernelGGL(k_real_out, dim3(grid_for(NL)), dim3(DT), 0, stream, C, coef + T.offsets[0] * nch, os,
+    hipLaunchKernelGGL(k_real_out, dim3(grid_for(NL)), dim3(DT), 0, ls, C, coef + T.offsets[0] * nch, os,
                        (float)(1.0 / (double)NL), NL);
     SH_LAUNCHED();
+    join_k();
   }
 
   void synthesize_channel(const float *coef, float *x) {
+    ls = stream;
     const int64_t is = nch;
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, stream, coef + T.offsets[0] * nch, is, C, NL,
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, ls, coef + T.offsets[0] * nch, is, C, NL,
                        1.0f);
     SH_LAUNCHED();
     fft3(-1, C, nl);
@@ -588,7 +626,9 @@ struct surfhip_s_plan {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
       const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
+      fork_k();
       for (int K = 0; K < 3; ++K) {
+        ls = kst[K];
         Dims3 hd = half_dims(s.n, K);
         float2 *S = Sbuf[K].p;
         lines(false, j, K, S, const_cast<float *>(coef), is);
@@ -599,30 +639,32 @@ struct surfhip_s_plan {
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
         for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
           const IRCStageT &st = s.stages[K][q];
-          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, stream, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
                              filt.p + st.off[2], filt.p + st.off[3], 1);
           SH_LAUNCHED();
         }
       }
+      ls = stream;
+      join_k();
       zero(X, NX);
       for (int K = 0; K < 3; ++K) {
         Dims3 hd = half_dims(s.n, K);
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
         for (int mirror = 0; mirror < 2; ++mirror) {
-          hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, stream, X, s.n[0], s.n[1], s.n[2],
+          hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, ls, X, s.n[0], s.n[1], s.n[2],
                              Sbuf[K].p, masks.p + s.mask_off[K], K, hd[0], hd[1], hd[2], (float)(1.0 / 3.0), mirror);
           SH_LAUNCHED();
         }
       }
       const PyrDev &pd = lev[j]->pd;
-      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, stream, X,
+      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls, X,
                          spectra[j + 1].p, pd, pd.D15);
       SH_LAUNCHED();
     }
     const Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
     fft3(1, spectra[0].p, n);
-    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, stream, spectra[0].p, x, (int64_t)nch,
+    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, ls, spectra[0].p, x, (int64_t)nch,
                        (float)(1.0 / (double)N), N);
     SH_LAUNCHED();
   }
@@ -630,11 +672,12 @@ struct surfhip_s_plan {
   // x = A^T c, the adjoint of the analysis (not its inverse: the transform
   // is a frame but not a tight one, src/transforms/surfacelet.jl:24-26).
   void adjoint_channel(const float *coef, float *x) {
+    ls = stream;
     const int64_t is = nch;
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, stream, coef + T.offsets[0] * nch, is, C, NL,
+    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, ls, coef + T.offsets[0] * nch, is, C, NL,
                        (float)(1.0 / (double)NL));
     SH_LAUNCHED();
     fft3(-1, C, nl);
@@ -642,7 +685,9 @@ struct surfhip_s_plan {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
       const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
+      fork_k();
       for (int K = 0; K < 3; ++K) {
+        ls = kst[K];
         Dims3 hd = half_dims(s.n, K);
         float2 *S = Sbuf[K].p;
         lines(false, j, K, S, const_cast<float *>(coef), is, 1);
@@ -653,30 +698,32 @@ struct surfhip_s_plan {
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
         for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
           const IRCStageT &st = s.stages[K][q];
-          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, stream, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
                              filt.p + st.off[0], filt.p + st.off[1], 2);
           SH_LAUNCHED();
         }
       }
+      ls = stream;
+      join_k();
       zero(X, NX);
       for (int K = 0; K < 3; ++K) {
         Dims3 hd = half_dims(s.n, K);
         Dims3 sb = subband_size(s, K);
         const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
-        hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, stream, X, s.n[0], s.n[1], s.n[2],
+        hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, ls, X, s.n[0], s.n[1], s.n[2],
                            Sbuf[K].p, masks.p + s.mask_off[K], K, hd[0], hd[1], hd[2], scale, 0);
         SH_LAUNCHED();
       }
       const PyrDev &pd = lev[j]->pd;
-      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, stream, X,
+      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls, X,
                          spectra[j + 1].p, pd, 1.0f / pd.D15);
       SH_LAUNCHED();
     }
     const Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
     fft3(1, spectra[0].p, n);
-    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, stream, spectra[0].p, x, (int64_t)nch, 1.0f, N);
+    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, ls, spectra[0].p, x, (int64_t)nch, 1.0f, N);
     SH_LAUNCHED();
   }
 };
@@ -733,6 +780,8 @@ int surfhip_s_plan_create(surfhip_s_plan **out, const surfhip_s_params *p, int32
     P->dtype = dtype;
     P->nch = dtype == SURFHIP_C64 ? 2 : 1;
     P->stream = (hipStream_t)stream;
+    P->ls = P->stream;
+    P->make_lanes();
     const DecTables &T = P->T;
     // FFT plans of every length used: level sizes, block sizes, coarse size.
     std::vector<int> lens;
