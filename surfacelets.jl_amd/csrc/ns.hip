@@ -243,6 +243,11 @@ struct surfhip_ns_plan {
   int dtype = SURFHIP_F32;
   int nch = 1;
   hipStream_t stream = nullptr;
+  // The two orientations of the factored path are independent until the
+  // synthesis sums meet: AX2 work runs on `aux`, forked from and joined back
+  // to `stream` with events (not while per-launch timing is on).
+  hipStream_t aux = nullptr;
+  hipEvent_t efork = nullptr, ejoin = nullptr;
   int64_t N = 0, H = 0;
   int n1 = 0, n2 = 0, n3 = 0, h1 = 0;
   FftPlan1D f1{}, f2{}, f3{};
@@ -319,6 +324,9 @@ struct surfhip_ns_plan {
       (void)hipEventDestroy(e.second.first);
       (void)hipEventDestroy(e.second.second);
     }
+    if (aux) (void)hipStreamDestroy(aux);
+    if (efork) (void)hipEventDestroy(efork);
+    if (ejoin) (void)hipEventDestroy(ejoin);
   }
 
   FusedCtx ctx() const {
@@ -655,6 +663,32 @@ struct surfhip_ns_plan {
           axis<3, -1, 3>(work.p, acc[ch].p, item0 + w0, nb, H, 0);
       }
   }
+  void make_aux() {
+    SH_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    SH_HIP(hipEventCreateWithFlags(&efork, hipEventDisableTiming));
+    SH_HIP(hipEventCreateWithFlags(&ejoin, hipEventDisableTiming));
+  }
+  void fork_aux() {
+    SH_HIP(hipEventRecord(efork, stream));
+    SH_HIP(hipStreamWaitEvent(aux, efork, 0));
+  }
+  void join_aux() {
+    SH_HIP(hipEventRecord(ejoin, aux));
+    SH_HIP(hipStreamWaitEvent(stream, ejoin, 0));
+  }
+  // Under stream capture the orientations stay on `stream`: a capture that
+  // forked to `aux` with these events crashed inside the runtime at capture
+  // end, so the captured sequence is the single-stream one.
+  bool concurrent() const {
+    if (!fused || !aux || timing) return false;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &st) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    return st == hipStreamCaptureStatusNone;
+  }
+
   void analyze(const float *x, float *coeffs) {
     forward_spectrum(x);
     if (!fused) {
@@ -663,6 +697,22 @@ struct surfhip_ns_plan {
       return;
     }
     const FusedCtx c = ctx();
+    if (concurrent()) {
+      FusedCtx ca = c;
+      ca.stream = aux;
+      for (int ch = 0; ch < nch; ++ch) {
+        fork_aux();
+        fops.p_ana[AX2](ca, X[ch].p, U[ch].p, dunits.p + ax_ubeg[AX2], ax_uend[AX2] - ax_ubeg[AX2], AX2);
+        fops.q_ana[AX2](ca, AX2, U[ch].p, djobs.p + ja_ax[AX2], jna_ax[AX2], coeffs + ch, N * nch, nch,
+                        1.0f / (float)N);
+        fops.p_ana[AX3](c, X[ch].p, U[ch].p, dunits.p + ax_ubeg[AX3], ax_uend[AX3] - ax_ubeg[AX3], AX3);
+        fops.q_ana[AX3](c, AX3, U[ch].p, djobs.p + ja_ax[AX3], jna_ax[AX3], coeffs + ch, N * nch, nch,
+                        1.0f / (float)N);
+        join_aux();
+      }
+      cached_group = -1;
+      return;
+    }
     for (int ch = 0; ch < nch; ++ch) {
       for (int o = 0; o < 2; ++o)
         timed(0, [&] {
@@ -714,6 +764,21 @@ struct surfhip_ns_plan {
       return;
     }
     const FusedCtx c = ctx();
+    if (concurrent()) {
+      FusedCtx ca = c;
+      ca.stream = aux;
+      const bool has2 = sl_n[AX2] > 0;
+      for (int ch = 0; ch < nch; ++ch) {
+        fork_aux();
+        fops.q_syn[AX2](ca, AX2, coeffs + ch, N * nch, nch, R[ch].p, djobs.p + js_ax[AX2], jns_ax[AX2]);
+        if (has2) fops.p_syn[AX2](ca, AX2, R[ch].p, dslist.p + sl_beg[AX2], sl_n[AX2], accf.p, 0, nullptr);
+        fops.q_syn[AX3](c, AX3, coeffs + ch, N * nch, nch, R[ch].p, djobs.p + js_ax[AX3], jns_ax[AX3]);
+        join_aux();
+        fops.p_syn[AX3](c, AX3, R[ch].p, dslist.p + sl_beg[AX3], sl_n[AX3], accf.p, has2 ? 1 : 0, TV.p);
+        fops.q_ana[AX3](c, AX3, TV.p, djobs.p + j_full, 1, x + ch, 0, nch, 1.0f / (float)N);
+      }
+      return;
+    }
     for (int ch = 0; ch < nch; ++ch) {
       for (int o = 0; o < 2; ++o)
         timed(2, [&] { fops.q_syn[o](c, o, coeffs + ch, N * nch, nch, R[ch].p, djobs.p + js_ax[o], jns_ax[o]); });
@@ -867,6 +932,7 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
     for (int ch = 0; ch < P->nch; ++ch) P->X[ch].alloc(P->H);
     if (P->fused) {
       P->build_fused();
+      P->make_aux();
     } else {
       for (int ch = 0; ch < P->nch; ++ch) P->acc[ch].alloc(P->H);
       // Batched general-path launches: up to nwork subbands per launch, as
