@@ -474,17 +474,18 @@ __global__ __launch_bounds__(pencil_threads<NP>()) void kP_fwd(const float2 *__r
 
 // Analysis plane pass, block (ip, job) for the single a-index p = p0:
 // out plane = scale * c2r_1(IFFT_c(U w_a,p)), U the slab plane.  Row k1 of
-// the plane is transformed by the R2 threads of slot k1; the results go to
-// LDS column-major (column stride CS) for the transform along k1, whose
-// outputs are stored straight from registers.
+// the plane is transformed by the R2 threads of slot k1 (slot 0 carries rows
+// 0 and n1/2, see below); the results go to LDS column-major (column stride
+// CS) for the transform along k1, whose outputs are stored straight from
+// registers.
 template <int NC> constexpr int col_stride() { return NC >= 64 ? 68 : 36; }
 template <int N1, int NC> constexpr int ana_slots() {
-  constexpr int a = (N1 / 2 + 1) * pencil_stride<NC>(), b = NC * col_stride<N1>(), c = (NC / 2) * pencil_stride<N1>();
+  constexpr int a = (N1 / 2) * pencil_stride<NC>(), b = NC * col_stride<N1>(), c = (NC / 2) * pencil_stride<N1>();
   constexpr int m = a > b ? a : b;
   return m > c ? m : c;
 }
 template <int N1, int NC> constexpr int ana_threads() {
-  constexpr int a = (N1 / 2 + 1) * FourStep<NC>::R2, b = (NC / 2) * FourStep<N1>::R2;
+  constexpr int a = (N1 / 2) * FourStep<NC>::R2, b = (NC / 2) * FourStep<N1>::R2;
   constexpr int m = a > b ? a : b;
   return (m + 63) / 64 * 64;
 }
@@ -506,14 +507,17 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
   const PlaneJob jb = jobs[blockIdx.y];
   const bool masked = jb.slab >= 0 && jb.group >= 0;
   const int slot = tid / RC2, j = tid % RC2;
-  const bool onslot = slot < H1;
+  const bool onslot = slot < HS;
   // Issue the slab and mask loads first; dead tiles are selected away below.
+  // Slot 0 also loads row HS: it transforms z = row 0 + i row HS, whose
+  // inverse along kc is real for both rows (their spectra are Hermitian in
+  // kc), so the real part is row 0's result and the imaginary part row HS's.
   const int p = jb.p0;
   const bool wunit = !masked || jb.pa0 < 0 || ns_unit_wedge(mv, true);
   const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
   const float2 *Ub = slabs + (size_t)(jb.slab < 0 ? 0 : jb.slab) * H + (size_t)ip * H1 * NC;
-  float2 u[RC1];
-  float wa[RC1];
+  float2 u[RC1], uh[RC1];
+  float wa[RC1], wh[RC1];
   const int row = onslot ? slot : 0;
 #pragma unroll
   for (int r = 0; r < RC1; ++r) {
@@ -521,12 +525,20 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     u[r] = Ub[o];
     wa[r] = wab[wunit ? 0u : o];
   }
+  if (slot == 0) {
+#pragma unroll
+    for (int r = 0; r < RC1; ++r) {
+      const unsigned o = (unsigned)(HS * NC + j + RC2 * r);
+      uh[r] = Ub[o];
+      wh[r] = wab[wunit ? 0u : o];
+    }
+  }
   RegFFT<N1, 1>::fill_twiddles(tw1, tw1g, tid, NTH);
   RegFFT<NC, 1>::fill_twiddles(twc, twcg, tid, NTH);
   for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
   __syncthreads();
-  const uint32_t lrow = onslot ? rl[slot] : 0u;
-  const bool act = lrow != 0u;
+  const uint32_t lrow = onslot ? rl[slot] : 0u, lhs = slot == 0 ? rl[HS] : 0u;
+  const bool act = (lrow | lhs) != 0u;
   float2 *pb = L + row * PS;
   {
     float2 v[RC1];
@@ -535,6 +547,14 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
       for (int r = 0; r < RC1; ++r) {
         const bool lv = ((lrow >> ((j + RC2 * r) / tpw)) & 1u) != 0;
         v[r] = lv ? cscale(u[r], wunit ? 1.0f : wa[r]) : make_float2(0.0f, 0.0f);
+      }
+      if (slot == 0) {
+#pragma unroll
+        for (int r = 0; r < RC1; ++r) {
+          const bool lv = ((lhs >> ((j + RC2 * r) / tpw)) & 1u) != 0;
+          const float2 b = lv ? cscale(uh[r], wunit ? 1.0f : wh[r]) : make_float2(0.0f, 0.0f);
+          v[r] = make_float2(v[r].x - b.y, v[r].y + b.x);
+        }
       }
       RegFFT<NC, 1>::phase1(v, j, twc);
 #pragma unroll
@@ -548,12 +568,20 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     }
     __syncthreads();
     if (onslot) {
-      // Rows without live tiles hold zeros.
+      // Rows without live tiles hold zeros; rows 0 and HS are real.
 #pragma unroll
       for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
-        for (int qq = 0; qq < RC2; ++qq)
-          L[((j * RCR + uu) + RC1 * qq) * CS + slot] = act ? v[uu * RC2 + qq] : make_float2(0.0f, 0.0f);
+        for (int qq = 0; qq < RC2; ++qq) {
+          const float2 z = act ? v[uu * RC2 + qq] : make_float2(0.0f, 0.0f);
+          float2 *lc = L + ((j * RCR + uu) + RC1 * qq) * CS;
+          if (slot == 0) {
+            lc[0] = make_float2(z.x, 0.0f);
+            lc[HS] = make_float2(z.y, 0.0f);
+          } else {
+            lc[slot] = z;
+          }
+        }
     }
     __syncthreads();
   }
