@@ -450,19 +450,20 @@ struct surfhip_ns_plan {
           fail(SURFHIP_ERR_ARG, "internal: wedge planes of a level are not evenly spaced");
       return jb;
     };
-    // Plane jobs.  Analysis: one job per subband (the plane blocks of one
-    // slab plane then run on one XCD, close in time, and share its reads in
-    // L2).  Synthesis: one job per slab, summing its nA subbands in order.
+    // Plane jobs.  Analysis: one job per slab, its nA subbands one after the
+    // other in each block, the longest jobs first: blocks of unequal length
+    // drift apart in time, so that some load while others compute.
+    // Synthesis: one job per slab, summing its nA subbands in order.
     jobs.clear();
     for (int o = 0; o < 2; ++o) {
       ja_ax[o] = (int)jobs.size();
       for (int g = 0; g <= G; ++g) {
         const GroupDev &gd = groups[g];
         if (gd.orient != o) continue;
-        for (int q = 0; q < gd.nB; ++q)
-          for (int p = 0; p < gd.nA; ++p)
-            jobs.push_back(mkjob(g, q, p, p + 1, 1, gd.first + p * gd.nB + q, 0));
+        for (int q = 0; q < gd.nB; ++q) jobs.push_back(mkjob(g, q, 0, gd.nA, 1, gd.first + q, gd.nB));
       }
+      std::stable_sort(jobs.begin() + ja_ax[o], jobs.end(),
+                       [](const PlaneJob &a, const PlaneJob &b) { return a.p1 - a.p0 > b.p1 - b.p0; });
       jna_ax[o] = (int)jobs.size() - ja_ax[o];
     }
     for (int o = 0; o < 2; ++o) {

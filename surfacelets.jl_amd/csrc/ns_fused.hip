@@ -83,6 +83,13 @@ template <class T> __device__ __forceinline__ T &at_b(T *base, unsigned bytes) {
   return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + bytes);
 }
 
+// a where bit is 1, else +0: the bit patterns are masked, so there is no
+// branch and whatever a holds (NaN included) is dropped.
+__device__ __forceinline__ float2 keep(float2 a, uint32_t bit) {
+  const uint32_t m = 0u - bit;
+  return make_float2(__uint_as_float(__float_as_uint(a.x) & m), __uint_as_float(__float_as_uint(a.y) & m));
+}
+
 __device__ __forceinline__ float2 fma2(float2 a, float s, float2 c) {
   return make_float2(__fmaf_rn(a.x, s, c.x), __fmaf_rn(a.y, s, c.y));
 }
@@ -472,12 +479,12 @@ __global__ __launch_bounds__(pencil_threads<NP>()) void kP_fwd(const float2 *__r
 // pbase = ip n1, cstr = n1 n2.  Plane a of a group is [k1][kc] in the pool
 // for both orientations (pair (1,2) for AX3, (1,3) for AX2).
 
-// Analysis plane pass, block (ip, job) for the single a-index p = p0:
-// out plane = scale * c2r_1(IFFT_c(U w_a,p)), U the slab plane.  Row k1 of
-// the plane is transformed by the R2 threads of slot k1 (slot 0 carries rows
-// 0 and n1/2, see below); the results go to LDS column-major (column stride
-// CS) for the transform along k1, whose outputs are stored straight from
-// registers.
+// Analysis plane pass, block (ip, job): for every a-index p in [p0, p1), the
+// plane of subband out0 + (p - p0) out_step = scale * c2r_1(IFFT_c(U w_a,p)),
+// U the slab plane.  Row k1 of the plane is transformed by the R2 threads of
+// slot k1 (slot 0 carries rows 0 and n1/2, see below); the results go to LDS
+// column-major (column stride CS) for the transform along k1, whose outputs
+// are stored straight from registers.
 template <int NC> constexpr int col_stride() { return NC >= 64 ? 68 : 36; }
 template <int N1, int NC> constexpr int ana_slots() {
   constexpr int a = (N1 / 2) * pencil_stride<NC>(), b = NC * col_stride<N1>(), c = (NC / 2) * pencil_stride<N1>();
@@ -500,158 +507,161 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
   constexpr int CS = col_stride<N1>();
   constexpr int NTH = ana_threads<N1, NC>();
   constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
-  __shared__ __align__(16) float2 L[ana_slots<N1, NC>()];
-  __shared__ float2 tw1[N1], twc[NC];
-  __shared__ uint32_t rl[H1];
-  const int tid = threadIdx.x, ip = blockIdx.x;
-  const PlaneJob jb = jobs[blockIdx.y];
-  const bool masked = jb.slab >= 0 && jb.group >= 0;
-  const int slot = tid / RC2, j = tid % RC2;
-  const bool onslot = slot < HS;
-  // Issue the slab and mask loads first; dead tiles are selected away below.
-  // Slot 0 also loads row HS: it transforms z = row 0 + i row HS, whose
-  // inverse along kc is real for both rows (their spectra are Hermitian in
-  // kc), so the real part is row 0's result and the imaginary part row HS's.
-  const int p = jb.p0;
-  const bool wunit = !masked || jb.pa0 < 0 || ns_unit_wedge(mv, true);
-  const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
-  const float2 *Ub = slabs + (size_t)(jb.slab < 0 ? 0 : jb.slab) * H + (size_t)ip * H1 * NC;
-  float2 u[RC1], uh[RC1];
-  float wa[RC1], wh[RC1];
-  const int row = onslot ? slot : 0;
-#pragma unroll
-  for (int r = 0; r < RC1; ++r) {
-    const unsigned o = (unsigned)(row * NC + j + RC2 * r);
-    u[r] = Ub[o];
-    wa[r] = wab[wunit ? 0u : o];
-  }
-  if (slot == 0) {
-#pragma unroll
-    for (int r = 0; r < RC1; ++r) {
-      const unsigned o = (unsigned)(HS * NC + j + RC2 * r);
-      uh[r] = Ub[o];
-      wh[r] = wab[wunit ? 0u : o];
-    }
-  }
-  RegFFT<N1, 1>::fill_twiddles(tw1, tw1g, tid, NTH);
-  RegFFT<NC, 1>::fill_twiddles(twc, twcg, tid, NTH);
-  for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
-  __syncthreads();
-  const uint32_t lrow = onslot ? rl[slot] : 0u, lhs = slot == 0 ? rl[HS] : 0u;
-  const bool act = (lrow | lhs) != 0u;
-  float2 *pb = L + row * PS;
-  {
-    float2 v[RC1];
-    if (act) {
-#pragma unroll
-      for (int r = 0; r < RC1; ++r) {
-        const bool lv = ((lrow >> ((j + RC2 * r) / tpw)) & 1u) != 0;
-        v[r] = lv ? cscale(u[r], wunit ? 1.0f : wa[r]) : make_float2(0.0f, 0.0f);
-      }
-      if (slot == 0) {
-#pragma unroll
-        for (int r = 0; r < RC1; ++r) {
-          const bool lv = ((lhs >> ((j + RC2 * r) / tpw)) & 1u) != 0;
-          const float2 b = lv ? cscale(uh[r], wunit ? 1.0f : wh[r]) : make_float2(0.0f, 0.0f);
-          v[r] = make_float2(v[r].x - b.y, v[r].y + b.x);
-        }
-      }
-      RegFFT<NC, 1>::phase1(v, j, twc);
-#pragma unroll
-      for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + j)] = v[s];
-    }
-    wave_sync();  // the exchange region of a row is private to its R2 lanes
-    if (act) {
-#pragma unroll
-      for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(j * RC1 + e)];
-      RegFFT<NC, 1>::phase2(v);
-    }
-    __syncthreads();
-    if (onslot) {
-      // Rows without live tiles hold zeros; rows 0 and HS are real.
-#pragma unroll
-      for (int uu = 0; uu < RCR; ++uu)
-#pragma unroll
-        for (int qq = 0; qq < RC2; ++qq) {
-          const float2 z = act ? v[uu * RC2 + qq] : make_float2(0.0f, 0.0f);
-          float2 *lc = L + ((j * RCR + uu) + RC1 * qq) * CS;
-          if (slot == 0) {
-            lc[0] = make_float2(z.x, 0.0f);
-            lc[HS] = make_float2(z.y, 0.0f);
-          } else {
-            lc[slot] = z;
-          }
-        }
-    }
-    __syncthreads();
-  }
-  // c2r along k1 on column pairs (2 pr, 2 pr + 1): z = a + i b through the
-  // Hermitian extension (imaginary parts of rows 0 and n1/2 ignored); rows
-  // that hold no data are zero.
   using F = RegFFT<N1, 1>;
   constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS;
   static_assert(R1 % 2 == 0 && R1 * R2 == 2 * HS, "row HS sits at element R1 / 2 of thread 0");
-  const int pr = tid / R2, jj = tid % R2;
-  const bool on = pr < NC / 2;
-  float2 w[R1];
-  if (on) {
-    const float2 *ca = L + (2 * pr) * CS, *cb = ca + CS;
-#pragma unroll
-    for (int r = 0; r < R1; ++r) {
-      // Row t = jj + R2 r: below HS = R1 R2 / 2 for r < R1 / 2, above it
-      // (the mirror N1 - t, conjugated) for r > R1 / 2; r = R1 / 2 holds HS
-      // on thread jj = 0 and mirrors elsewhere.  Rows 0 and HS are real.
-      const int t = jj + R2 * r;
-      const bool mir = 2 * r > R1 || (2 * r == R1 && jj > 0);
-      const bool real = (r == 0 || 2 * r == R1) && jj == 0;
-      float2 A = ca[mir ? N1 - t : t], B = cb[mir ? N1 - t : t];
-      if (mir) {
-        A = conjf2(A);
-        B = conjf2(B);
-      }
-      if (real) {
-        A.y = 0.0f;
-        B.y = 0.0f;
-      }
-      w[r] = make_float2(A.x - B.y, A.y + B.x);
-    }
-    F::phase1(w, jj, tw1);
-  }
-  __syncthreads();
-  float2 *qb = L + (on ? pr : 0) * PP;
-  if (on) {
-#pragma unroll
-    for (int s = 0; s < R1; ++s) qb[xpos<R1>(s * R2 + jj)] = w[s];
-  }
-  wave_sync();  // the exchange region of a column pair is private to its R2 lanes
-  if (!on) return;
-#pragma unroll
-  for (int e = 0; e < R1; ++e) w[e] = qb[xpos<R1>(jj * R1 + e)];
-  F::phase2(w);
-  // Thread (pr, jj) holds columns 2 pr (.x) and 2 pr + 1 (.y) at
-  // i1 = (jj ROWS + u) + R1 q.
+  __shared__ __align__(16) float2 L[ana_slots<N1, NC>()];
+  __shared__ float2 tw1[N1], twc[NC], UW[NC];
+  __shared__ uint32_t rl[H1];
+  const int ip = blockIdx.x;
+  const PlaneJob jb = jobs[blockIdx.y];
+  const bool masked = jb.slab >= 0 && jb.group >= 0;
+  const bool wunit = !masked || jb.pa0 < 0 || ns_unit_wedge(mv, true);
+  const float2 *Ub = slabs + (size_t)(jb.slab < 0 ? 0 : jb.slab) * H + (size_t)ip * H1 * NC;
   const int pbase = OR == AX3 ? ip * n1n2 : ip * N1;
   const int cstr = OR == AX3 ? N1 : n1n2;
-  float *dst = out + (int64_t)jb.out0 * substride + (int64_t)pbase * os;
-  if (ROWS == 2 && os == 1 && (reinterpret_cast<uintptr_t>(dst) & 7) == 0 && (cstr & 1) == 0) {
-    float *d0 = dst + (unsigned)((2 * pr) * cstr), *d1 = d0 + cstr;
+  for (int p = jb.p0; p < jb.p1; ++p) {
+    // Index math is redone per subband from an opaque copy of the thread
+    // index, so that nothing but the loop counter is live across the loop.
+    const int tid = opaque((int)threadIdx.x);
+    const int slot = tid / RC2, j = tid % RC2;
+    const bool onslot = slot < HS;
+    const int row = onslot ? slot : 0;
+    // Issue the slab and mask loads first; dead tiles are selected away
+    // below.  Row HS, scaled by its wedge-plane row, goes to UW (thread
+    // kc < NC takes column kc) for slot 0, which transforms z = row 0 + i
+    // row HS: the inverse along kc of each row is real (their spectra are
+    // Hermitian in kc), so the real part is row 0's result and the imaginary
+    // part row HS's.
+    const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
+    float2 u[RC1];
+    float wa[RC1];
 #pragma unroll
-    for (int q = 0; q < R2; ++q) {
-      const int i1 = jj * 2 + R1 * q;
-      const float2 z0 = w[q], z1 = w[R2 + q];
-      *reinterpret_cast<float2 *>(d0 + i1) = make_float2(z0.x * scale, z1.x * scale);
-      *reinterpret_cast<float2 *>(d1 + i1) = make_float2(z0.y * scale, z1.y * scale);
+    for (int r = 0; r < RC1; ++r) {
+      const unsigned o = (unsigned)(row * NC + j + RC2 * r);
+      u[r] = Ub[o];
+      wa[r] = wab[wunit ? 0u : o];
     }
-  } else {
+    static_assert(NTH >= NC, "one thread per column of row HS");
+    if (tid < NC) {
+      const float2 uh = Ub[(unsigned)(HS * NC + tid)];
+      UW[tid] = wunit ? uh : cscale(uh, wab[(unsigned)(HS * NC + tid)]);
+    }
+    if (p == jb.p0) {
+      RegFFT<N1, 1>::fill_twiddles(tw1, tw1g, tid, NTH);
+      RegFFT<NC, 1>::fill_twiddles(twc, twcg, tid, NTH);
+      for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
+    }
+    __syncthreads();  // also: the previous subband is done with L
+    const uint32_t lrow = onslot ? rl[slot] : 0u, lhs = slot == 0 ? rl[HS] : 0u;
+    const bool act = (lrow | lhs) != 0u;
+    float2 *pb = L + row * PS;
+    {
+      float2 v[RC1];
+      if (act) {
 #pragma unroll
-    for (int uu = 0; uu < ROWS; ++uu)
+        for (int r = 0; r < RC1; ++r)
+          v[r] = keep(cscale(u[r], wunit ? 1.0f : wa[r]), (lrow >> ((j + RC2 * r) / tpw)) & 1u);
+        if (slot == 0) {
 #pragma unroll
-      for (int q = 0; q < R2; ++q) {
-        const int i1 = (jj * ROWS + uu) + R1 * q;
-        const float2 z = w[uu * R2 + q];
-        dst[(int64_t)((2 * pr) * cstr + i1) * os] = z.x * scale;
-        dst[(int64_t)((2 * pr + 1) * cstr + i1) * os] = z.y * scale;
+          for (int r = 0; r < RC1; ++r) {
+            const int kc = j + RC2 * r;
+            const float2 b = keep(UW[kc], (lhs >> (kc / tpw)) & 1u);
+            v[r] = make_float2(v[r].x - b.y, v[r].y + b.x);
+          }
+        }
+        RegFFT<NC, 1>::phase1(v, j, twc);
+#pragma unroll
+        for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + j)] = v[s];
       }
+      wave_sync();  // the exchange region of a row is private to its R2 lanes
+      if (act) {
+#pragma unroll
+        for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(j * RC1 + e)];
+        RegFFT<NC, 1>::phase2(v);
+      }
+      __syncthreads();
+      if (onslot) {
+        // Rows without live tiles hold zeros; rows 0 and HS are real.
+#pragma unroll
+        for (int uu = 0; uu < RCR; ++uu)
+#pragma unroll
+          for (int qq = 0; qq < RC2; ++qq) {
+            const float2 z = act ? v[uu * RC2 + qq] : make_float2(0.0f, 0.0f);
+            float2 *lc = L + ((j * RCR + uu) + RC1 * qq) * CS;
+            if (slot == 0) {
+              lc[0] = make_float2(z.x, 0.0f);
+              lc[HS] = make_float2(z.y, 0.0f);
+            } else {
+              lc[slot] = z;
+            }
+          }
+      }
+      __syncthreads();
+    }
+    // c2r along k1 on column pairs (2 pr, 2 pr + 1): z = a + i b through the
+    // Hermitian extension; rows that hold no data are zero.
+    const int pr = tid / R2, jj = tid % R2;
+    const bool on = pr < NC / 2;
+    float2 w[R1];
+    if (on) {
+      const float2 *ca = L + (2 * pr) * CS, *cb = ca + CS;
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        // Row t = jj + R2 r: below HS = R1 R2 / 2 for r < R1 / 2, above it
+        // (the mirror N1 - t, conjugated) for r > R1 / 2; r = R1 / 2 holds
+        // HS on thread jj = 0 and mirrors elsewhere.  Rows 0 and HS are real.
+        const int t = jj + R2 * r;
+        const bool mir = 2 * r > R1 || (2 * r == R1 && jj > 0);
+        const bool real = (r == 0 || 2 * r == R1) && jj == 0;
+        float2 A = ca[mir ? N1 - t : t], B = cb[mir ? N1 - t : t];
+        if (mir) {
+          A = conjf2(A);
+          B = conjf2(B);
+        }
+        if (real) {
+          A.y = 0.0f;
+          B.y = 0.0f;
+        }
+        w[r] = make_float2(A.x - B.y, A.y + B.x);
+      }
+      F::phase1(w, jj, tw1);
+    }
+    __syncthreads();
+    float2 *qb = L + (on ? pr : 0) * PP;
+    if (on) {
+#pragma unroll
+      for (int s = 0; s < R1; ++s) qb[xpos<R1>(s * R2 + jj)] = w[s];
+    }
+    wave_sync();  // the exchange region of a column pair is private to its R2 lanes
+    if (on) {
+#pragma unroll
+      for (int e = 0; e < R1; ++e) w[e] = qb[xpos<R1>(jj * R1 + e)];
+      F::phase2(w);
+      // Thread (pr, jj) holds columns 2 pr (.x) and 2 pr + 1 (.y) at
+      // i1 = (jj ROWS + u) + R1 q.
+      float *dst = out + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * os;
+      if (ROWS == 2 && os == 1 && (reinterpret_cast<uintptr_t>(dst) & 7) == 0 && (cstr & 1) == 0) {
+        float *d0 = dst + (unsigned)((2 * pr) * cstr), *d1 = d0 + cstr;
+#pragma unroll
+        for (int q = 0; q < R2; ++q) {
+          const int i1 = jj * 2 + R1 * q;
+          const float2 z0 = w[q], z1 = w[R2 + q];
+          *reinterpret_cast<float2 *>(d0 + i1) = make_float2(z0.x * scale, z1.x * scale);
+          *reinterpret_cast<float2 *>(d1 + i1) = make_float2(z0.y * scale, z1.y * scale);
+        }
+      } else {
+#pragma unroll
+        for (int uu = 0; uu < ROWS; ++uu)
+#pragma unroll
+          for (int q = 0; q < R2; ++q) {
+            const int i1 = (jj * ROWS + uu) + R1 * q;
+            const float2 z = w[uu * R2 + q];
+            dst[(int64_t)((2 * pr) * cstr + i1) * os] = z.x * scale;
+            dst[(int64_t)((2 * pr + 1) * cstr + i1) * os] = z.y * scale;
+          }
+      }
+    }
   }
 }
 
