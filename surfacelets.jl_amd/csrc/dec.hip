@@ -4,10 +4,11 @@
 //   X = FFT_3(x) (complex, full grid);
 //   per pyramid level j (finest first): split X into its highpass (in place)
 //   and the resampled lowpass (fourier_pyramid.jl:108-118); then per
-//   hourglass K: S_K = X[half_K] V_K / prod(subband size), the IRC analysis
-//   stages (irc.jl:127-134), inverse FFTs along the two other axes within each
-//   wedge block, and a c2r along K that writes each block's real subband into
-//   the flat coefficient vector (surfacelet_scale.jl:200-213);
+//   hourglass K: S_K = X[half_K] V_K / prod(subband size) and the IRC analysis
+//   stages (irc.jl:127-134), one pass (k_irc_group), inverse FFTs along the two
+//   other axes within each wedge block, and a c2r along K that writes each
+//   block's real subband into the flat coefficient vector
+//   (surfacelet_scale.jl:200-213);
 //   coarse subband = real(IFFT_3(lowpass)).
 // Synthesis mirrors it (surfacelet.jl:384-396, surfacelet_scale.jl:220-257).
 // Every index map (pyramid boxes, block ranges, Hermitian mirror, subband
@@ -66,7 +67,6 @@ __global__ __launch_bounds__(DT) void k_c2c(const float2 *in, float2 *out, int m
       base0 = lin3(u0, s * seg, i3, m1, m2);
       sa = m1;
     } else {
-      const int ns = m3 / seg;
       const int i2 = rest % m2, s = rest / m2;
       base0 = lin3(u0, i2, s * seg, m1, m2);
       sa = (int64_t)m1 * m2;
@@ -398,33 +398,147 @@ __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, 
   }
 }
 
-// X += c M S on the half of axis K and X[-w] += c M conj(S) on its strictly
-// interior mirror, index i -> (n - i) mod n (surfacelet_scale.jl:246-257).
-__global__ void k_hermitian_add(float2 *__restrict__ X, int n1, int n2, int n3, const float2 *__restrict__ S,
-                                const float *__restrict__ M, int K, int h1, int h2, int h3, float c, int mirror) {
-  const int n[3] = {n1, n2, n3};
-  const int64_t tot = (int64_t)h1 * h2 * h3;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-    int i[3];
-    i[0] = e % h1;
-    const int64_t r = e / h1;
-    i[1] = r % h2;
-    i[2] = r / h2;
-    const float2 s = S[e];
-    const float w = c * M[e];
-    if (!mirror) {
-      const int64_t xi = lin3(i[0], i[1], i[2], n1, n2);
-      const float2 x = X[xi];
-      X[xi] = make_float2(x.x + w * s.x, x.y + w * s.y);
+// All IRC stages of hourglass K at one scale in one pass.  The stages along
+// axis m (level le = 1..L, block size b = h_m >> le) pair elements at
+// distance b, so the 2^L elements i_m = r + a (h_m >> L), a < 2^L, of one
+// residue r are closed under them: stage le acts on bit L - le of a.  A
+// thread holds such a group for both transverse axes m1 < m2 (group index
+// g = a + 2^L1 b; stage le of m2 acts on bit L1 + L2 - le) and one index
+// along K, and applies the stages in registers with the arithmetic of
+// k_irc: mode 0 analysis in stage order, on X[half_K] V_K scale loaded here
+// (k_masked); modes 1 (synthesis) and 2 (adjoint) in reverse stage order, in
+// place on S.
+struct IrcGroup {
+  const float2 *f0[8], *f1[8];  // per stage in stage order: m1 levels 1..L1, then m2 levels 1..L2
+};
+
+template <int L1, int L2>
+__global__ __launch_bounds__(DT) void k_irc_group(const float2 *__restrict__ X, int n1, int n2,
+                                                  const float *__restrict__ V, float scale, float2 *__restrict__ S,
+                                                  int h1, int h2, int h3, int K, int m1, int m2, IrcGroup F,
+                                                  int mode) {
+  constexpr int T = L1 + L2, NG = 1 << T;
+  const int hd[3] = {h1, h2, h3};
+  const int B1 = hd[m1] >> L1, B2 = hd[m2] >> L2, nKh = hd[K];
+  const int64_t ngroups = (int64_t)hd[K] * B1 * B2;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ngroups; q += (int64_t)gridDim.x * blockDim.x) {
+    // The axis that is contiguous in memory varies fastest: K when K = 0,
+    // else m1 (= 0).
+    int iK, r1, r2;
+    if (K == 0) {
+      iK = (int)(q % nKh);
+      const int64_t t = q / nKh;
+      r1 = (int)(t % B1);
+      r2 = (int)(t / B1);
     } else {
-      const int iK = i[K];
-      if (iK < 1 || iK > n[K] / 2 - 1) continue;
-      int j[3];
-      for (int d = 0; d < 3; ++d) j[d] = d == K ? n[K] - iK : (n[d] - i[d]) % n[d];
-      const int64_t xj = lin3(j[0], j[1], j[2], n1, n2);
-      const float2 x = X[xj];
-      X[xj] = make_float2(x.x + w * s.x, x.y - w * s.y);
+      r1 = (int)(q % B1);
+      const int64_t t = q / B1;
+      iK = (int)(t % nKh);
+      r2 = (int)(t / nKh);
     }
+    auto coords = [&](int g, int idx[3]) {
+      idx[K] = iK;
+      idx[m1] = r1 + (g & ((1 << L1) - 1)) * B1;
+      idx[m2] = r2 + (g >> L1) * B2;
+    };
+    float2 v[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      int idx[3];
+      coords(g, idx);
+      const int64_t e = lin3(idx[0], idx[1], idx[2], h1, h2);
+      if (mode == 0) {
+        const float2 x = X[lin3(idx[0], idx[1], idx[2], n1, n2)];
+        const float m = V[e] * scale;
+        v[g] = make_float2(x.x * m, x.y * m);
+      } else {
+        v[g] = S[e];
+      }
+    }
+    // Stage s (stage order) acts on bit sb(s) of g along axis m(s); the
+    // coordinate of element g along that axis is c(g).
+    auto stage = [&](int s) {
+      const bool on1 = s < L1;
+      const int bit = on1 ? L1 - 1 - s : T - 1 - (s - L1);
+      const float2 *f0 = F.f0[s], *f1 = F.f1[s];
+#pragma unroll
+      for (int gA = 0; gA < NG; ++gA) {
+        if (gA & (1 << bit)) continue;
+        const int gB = gA | (1 << bit);
+        const int cA = on1 ? r1 + (gA & ((1 << L1) - 1)) * B1 : r2 + (gA >> L1) * B2;
+        const int cB = on1 ? r1 + (gB & ((1 << L1) - 1)) * B1 : r2 + (gB >> L1) * B2;
+        const float2 A = v[gA], B = v[gB];
+        const float2 f0A = f0[iK + (int64_t)nKh * cA], f0B = f0[iK + (int64_t)nKh * cB];
+        const float2 f1A = f1[iK + (int64_t)nKh * cA], f1B = f1[iK + (int64_t)nKh * cB];
+        if (mode == 0) {
+          v[gA] = cadd(cmul(f0A, A), cmul(f0B, B));
+          v[gB] = cadd(cmul(f1A, A), cmul(f1B, B));
+        } else if (mode == 1) {
+          v[gA] = cadd(cmul(f0A, A), cmul(f1A, B));
+          v[gB] = cadd(cmul(f0B, A), cmul(f1B, B));
+        } else {
+          v[gA] = cadd(cmul(conjf2(f0A), A), cmul(conjf2(f1A), B));
+          v[gB] = cadd(cmul(conjf2(f0B), A), cmul(conjf2(f1B), B));
+        }
+      }
+    };
+    if (mode == 0) {
+#pragma unroll
+      for (int s = 0; s < T; ++s) stage(s);
+    } else {
+#pragma unroll
+      for (int s = T - 1; s >= 0; --s) stage(s);
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      int idx[3];
+      coords(g, idx);
+      S[lin3(idx[0], idx[1], idx[2], h1, h2)] = v[g];
+    }
+  }
+}
+
+// X = sum over K of c_K M_K S_K on the half of axis K, plus (mirror) c_K M_K
+// conj(S_K) on its strictly interior mirror, index i -> (n - i) mod n: the
+// sum of one scale (surfacelet_scale.jl:246-257) as one write of X: for each
+// element the contributions are added in K order, and at most one of the two
+// (direct, mirrored) per K reaches it.
+struct HermSum {
+  const float2 *S[3];
+  const float *M[3];
+  float c[3];
+};
+
+__global__ void k_hermitian_sum(float2 *__restrict__ X, int n1, int n2, int n3, HermSum H, int mirror) {
+  const int n[3] = {n1, n2, n3};
+  const int64_t tot = (int64_t)n1 * n2 * n3;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    int j[3];
+    j[0] = (int)(e % n1);
+    const int64_t r = e / n1;
+    j[1] = (int)(r % n2);
+    j[2] = (int)(r / n2);
+    float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int K = 0; K < 3; ++K) {
+      const int hK = n[K] / 2 + 1;
+      int h[3] = {n1, n2, n3};
+      h[K] = hK;
+      if (j[K] < hK) {
+        const int64_t s = lin3(j[0], j[1], j[2], h[0], h[1]);
+        const float w = H.c[K] * H.M[K][s];
+        const float2 v = H.S[K][s];
+        acc = make_float2(acc.x + w * v.x, acc.y + w * v.y);
+      } else if (mirror) {
+        int i[3];
+        for (int d = 0; d < 3; ++d) i[d] = d == K ? n[K] - j[K] : (n[d] - j[d]) % n[d];
+        const int64_t s = lin3(i[0], i[1], i[2], h[0], h[1]);
+        const float w = H.c[K] * H.M[K][s];
+        const float2 v = H.S[K][s];
+        acc = make_float2(acc.x + w * v.x, acc.y - w * v.y);
+      }
+    }
+    X[e] = acc;
   }
 }
 
@@ -525,6 +639,71 @@ struct surfhip_s_plan {
     hipLaunchKernelGGL(k_dzero, dim3(grid_for(n)), dim3(DT), 0, ls, p, n);
     SH_LAUNCHED();
   }
+  static void transverse(int K, int &m1, int &m2) {
+    m1 = K == 0 ? 1 : 0;
+    m2 = K == 2 ? 1 : 2;
+  }
+  // Hourglasses whose IRC groups hold at most 16 elements run their stages
+  // (and, in analysis, the ritf mask) as one k_irc_group pass.
+  static bool grouped(const DecScaleT &s, int K) {
+    int m1, m2;
+    transverse(K, m1, m2);
+    return s.lv[K][m1] + s.lv[K][m2] <= 4;
+  }
+  // mode 0: S = IRC_analysis(X[half_K] V scale); 1: S = IRC_synthesis(S);
+  // 2: S = IRC_analysis^H(S).
+  void irc_group(const DecScaleT &s, int K, int mode, const float2 *X, const float *V, float scale, float2 *S) {
+    int m1, m2;
+    transverse(K, m1, m2);
+    const int L1 = s.lv[K][m1], L2 = s.lv[K][m2];
+    const Dims3 hd = half_dims(s.n, K);
+    IrcGroup F{};
+    const auto &stg = s.stages[K];
+    if ((int)stg.size() != L1 + L2) fail(SURFHIP_ERR_DEVICE, "internal: IRC stage count");
+    for (int q = 0; q < L1 + L2; ++q) {
+      F.f0[q] = filt.p + stg[q].off[mode == 1 ? 2 : 0];
+      F.f1[q] = filt.p + stg[q].off[mode == 1 ? 3 : 1];
+    }
+    const int64_t ng = (int64_t)hd[K] * (hd[m1] >> L1) * (hd[m2] >> L2);
+    const dim3 grid(grid_for(ng)), block(DT);
+#define SH_IRCG(A, B)                                                                                                \
+  hipLaunchKernelGGL((k_irc_group<A, B>), grid, block, 0, ls, X, s.n[0], s.n[1], V, scale, S, hd[0], hd[1], hd[2], K, \
+                     m1, m2, F, mode)
+    switch (L1 * 5 + L2) {
+      case 0: SH_IRCG(0, 0); break;
+      case 1: SH_IRCG(0, 1); break;
+      case 2: SH_IRCG(0, 2); break;
+      case 3: SH_IRCG(0, 3); break;
+      case 4: SH_IRCG(0, 4); break;
+      case 5: SH_IRCG(1, 0); break;
+      case 6: SH_IRCG(1, 1); break;
+      case 7: SH_IRCG(1, 2); break;
+      case 8: SH_IRCG(1, 3); break;
+      case 10: SH_IRCG(2, 0); break;
+      case 11: SH_IRCG(2, 1); break;
+      case 12: SH_IRCG(2, 2); break;
+      case 15: SH_IRCG(3, 0); break;
+      case 16: SH_IRCG(3, 1); break;
+      case 20: SH_IRCG(4, 0); break;
+      default: fail(SURFHIP_ERR_DEVICE, "internal: IRC group size");
+    }
+#undef SH_IRCG
+    SH_LAUNCHED();
+  }
+  // X of scale j = the hourglass spectra summed back (with their Hermitian
+  // mirrors when `mirror`), each scaled by c[K] and its ritf mask.
+  void hermitian_sum(int j, float2 *X, const float c[3], int mirror) {
+    const DecScaleT &s = T.scales[j];
+    HermSum H{};
+    for (int K = 0; K < 3; ++K) {
+      H.S[K] = Sbuf[K].p;
+      H.M[K] = masks.p + s.mask_off[K];
+      H.c[K] = c[K];
+    }
+    const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
+    hipLaunchKernelGGL(k_hermitian_sum, dim3(grid_for(NX)), dim3(DT), 0, ls, X, s.n[0], s.n[1], s.n[2], H, mirror);
+    SH_LAUNCHED();
+  }
   BlockMap block_map(int j, int K) const {
     const DecScaleT &s = T.scales[j];
     Dims3 counts, bsize;
@@ -586,13 +765,17 @@ struct surfhip_s_plan {
         const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
         float2 *S = Sbuf[K].p;
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
-        hipLaunchKernelGGL(k_masked, dim3(grid_for(HN)), dim3(DT), 0, ls, spectra[j].p, s.n[0], s.n[1], S,
-                           masks.p + s.mask_off[K], hd[0], hd[1], hd[2], scale);
-        SH_LAUNCHED();
-        for (const IRCStageT &st : s.stages[K]) {
-          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
-                             filt.p + st.off[0], filt.p + st.off[1], 0);
+        if (grouped(s, K)) {
+          irc_group(s, K, 0, spectra[j].p, masks.p + s.mask_off[K], scale, S);
+        } else {
+          hipLaunchKernelGGL(k_masked, dim3(grid_for(HN)), dim3(DT), 0, ls, spectra[j].p, s.n[0], s.n[1], S,
+                             masks.p + s.mask_off[K], hd[0], hd[1], hd[2], scale);
           SH_LAUNCHED();
+          for (const IRCStageT &st : s.stages[K]) {
+            hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+                               filt.p + st.off[0], filt.p + st.off[1], 0);
+            SH_LAUNCHED();
+          }
         }
         Dims3 counts, bsize;
         scale_blocks(s, K, counts, bsize);
@@ -625,7 +808,6 @@ struct surfhip_s_plan {
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
-      const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
       fork_k();
       for (int K = 0; K < 3; ++K) {
         ls = kst[K];
@@ -637,24 +819,22 @@ struct surfhip_s_plan {
         for (int d = 0; d < 3; ++d)
           if (d != K) launch_c2c(d, -1, S, hd, bsize[d]);
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
-        for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
-          const IRCStageT &st = s.stages[K][q];
-          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
-                             filt.p + st.off[2], filt.p + st.off[3], 1);
-          SH_LAUNCHED();
+        if (grouped(s, K)) {
+          if (!s.stages[K].empty()) irc_group(s, K, 1, nullptr, nullptr, 1.0f, S);
+        } else {
+          for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
+            const IRCStageT &st = s.stages[K][q];
+            hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m,
+                               st.b, filt.p + st.off[2], filt.p + st.off[3], 1);
+            SH_LAUNCHED();
+          }
         }
       }
       ls = stream;
       join_k();
-      zero(X, NX);
-      for (int K = 0; K < 3; ++K) {
-        Dims3 hd = half_dims(s.n, K);
-        const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
-        for (int mirror = 0; mirror < 2; ++mirror) {
-          hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, ls, X, s.n[0], s.n[1], s.n[2],
-                             Sbuf[K].p, masks.p + s.mask_off[K], K, hd[0], hd[1], hd[2], (float)(1.0 / 3.0), mirror);
-          SH_LAUNCHED();
-        }
+      {
+        const float c[3] = {(float)(1.0 / 3.0), (float)(1.0 / 3.0), (float)(1.0 / 3.0)};
+        hermitian_sum(j, X, c, 1);
       }
       const PyrDev &pd = lev[j]->pd;
       hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls, X,
@@ -684,7 +864,6 @@ struct surfhip_s_plan {
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
-      const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
       fork_k();
       for (int K = 0; K < 3; ++K) {
         ls = kst[K];
@@ -696,24 +875,26 @@ struct surfhip_s_plan {
         for (int d = 0; d < 3; ++d)
           if (d != K) launch_c2c(d, -1, S, hd, bsize[d]);
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
-        for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
-          const IRCStageT &st = s.stages[K][q];
-          hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
-                             filt.p + st.off[0], filt.p + st.off[1], 2);
-          SH_LAUNCHED();
+        if (grouped(s, K)) {
+          if (!s.stages[K].empty()) irc_group(s, K, 2, nullptr, nullptr, 1.0f, S);
+        } else {
+          for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
+            const IRCStageT &st = s.stages[K][q];
+            hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m,
+                               st.b, filt.p + st.off[0], filt.p + st.off[1], 2);
+            SH_LAUNCHED();
+          }
         }
       }
       ls = stream;
       join_k();
-      zero(X, NX);
-      for (int K = 0; K < 3; ++K) {
-        Dims3 hd = half_dims(s.n, K);
-        Dims3 sb = subband_size(s, K);
-        const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
-        const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
-        hipLaunchKernelGGL(k_hermitian_add, dim3(grid_for(HN)), dim3(DT), 0, ls, X, s.n[0], s.n[1], s.n[2],
-                           Sbuf[K].p, masks.p + s.mask_off[K], K, hd[0], hd[1], hd[2], scale, 0);
-        SH_LAUNCHED();
+      {
+        float c[3];
+        for (int K = 0; K < 3; ++K) {
+          const Dims3 sb = subband_size(s, K);
+          c[K] = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
+        }
+        hermitian_sum(j, X, c, 0);
       }
       const PyrDev &pd = lev[j]->pd;
       hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls, X,
