@@ -282,21 +282,48 @@ __device__ __forceinline__ int64_t half_elem(const BlockMap &M, const int o[3], 
   return lin3(i[0], i[1], i[2], M.h[0], M.h[1]);
 }
 
+// Both element maps are linear in t along a line: the offsets of a block's
+// G lines at t = 0 go to LDS once, and element t of line g is base + t stride.
+constexpr int LINES_MAX = 16;
+struct LineBases {
+  int64_t s[LINES_MAX], c[LINES_MAX];  // half-spectrum and coefficient offsets at t = 0
+};
+__device__ __forceinline__ void line_bases(const BlockMap &M, int64_t l0, int valid, LineBases &B, int tid) {
+  if (tid < valid) {
+    int o[3];
+    line_index(M, l0 + tid, o);
+    B.s[tid] = half_elem(M, o, 0);
+    B.c[tid] = block_elem(M, o, 0);
+  }
+}
+__device__ __forceinline__ int64_t half_stride(const BlockMap &M) {
+  return M.K == 0 ? 1 : (M.K == 1 ? (int64_t)M.h[0] : (int64_t)M.h[0] * M.h[1]);
+}
+__device__ __forceinline__ int64_t block_stride(const BlockMap &M) {
+  return M.K == 0 ? 1 : (M.K == 1 ? (int64_t)M.sb[0] : (int64_t)M.sb[0] * M.sb[1]);
+}
+
 // Subband lines = c2r along K of S (unnormalised), scattered into the
-// coefficients (element stride os).  G lines per block.
+// coefficients (element stride os).  G lines per block, two per complex FFT:
+// lines 2f and 2f + 1 have Hermitian spectra A and B, so the inverse of
+// A + iB is line 2f + i line 2f + 1.
 __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, float *__restrict__ out, int64_t os,
                                                  BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
   extern __shared__ float2 smem[];
-  const int nK = M.nK, hK = M.h[M.K];
+  const int nK = M.nK, hK = M.h[M.K], F = (G + 1) / 2;
   float2 *tw = smem;
   float2 *st = tw + nK;
   float2 *a = st + (size_t)hK * G;
-  float2 *b = a + (size_t)nK * G;
+  float2 *b = a + (size_t)nK * F;
   const int tid = threadIdx.x;
   for (int k = tid; k < nK; k += DT) tw[k] = twg[pl.tw + k];
   const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
   const int64_t l0 = (int64_t)blockIdx.x * G;
   const int valid = (int)min((int64_t)G, nlines - l0);
+  __shared__ LineBases LB;
+  line_bases(M, l0, valid, LB, tid);
+  const int64_t sS = half_stride(M), sC = block_stride(M);
+  __syncthreads();
   for (int idx = tid; idx < hK * G; idx += DT) {
     int t, g;
     if (M.K == 0) {
@@ -306,18 +333,13 @@ __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, 
       t = idx / G;
       g = idx - t * G;
     }
-    float2 v = make_float2(0.0f, 0.0f);
-    if (g < valid) {
-      int o[3];
-      line_index(M, l0 + g, o);
-      v = S[half_elem(M, o, t)];
-    }
-    st[t * G + g] = v;
+    st[t * G + g] = g < valid ? S[LB.s[g] + t * sS] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
   const bool even = (nK & 1) == 0;
-  for (int idx = tid; idx < nK * G; idx += DT) {
-    const int t = idx / G, g = idx - t * G;
+  // Bin t of line g through the Hermitian extension (bins 0 and nK/2 real).
+  auto herm = [&](int t, int g) {
+    if (g >= G) return make_float2(0.0f, 0.0f);
     float2 z;
     if (t < hK) {
       z = st[t * G + g];
@@ -325,10 +347,15 @@ __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, 
     } else {
       z = conjf2(st[(nK - t) * G + g]);
     }
-    a[idx] = z;
+    return z;
+  };
+  for (int idx = tid; idx < nK * F; idx += DT) {
+    const int t = idx / F, f = idx - t * F;
+    const float2 A = herm(t, 2 * f), B = herm(t, 2 * f + 1);
+    a[idx] = make_float2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  float2 *res = lds_fft<1>(a, b, pl, G, tw, tid, DT);
+  float2 *res = lds_fft<1>(a, b, pl, F, tw, tid, DT);
   for (int idx = tid; idx < nK * G; idx += DT) {
     int t, g;
     if (M.K == 0) {
@@ -339,47 +366,48 @@ __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, 
       g = idx - t * G;
     }
     if (g >= valid) continue;
-    int o[3];
-    line_index(M, l0 + g, o);
-    out[block_elem(M, o, t) * os] = res[t * G + g].x;
+    const float2 z = res[t * F + g / 2];
+    out[(LB.c[g] + t * sC) * os] = (g & 1) ? z.y : z.x;
   }
 }
 
 // S lines = r2c along K of the subbands' lines (the first nK/2+1 bins).
 // With `dbl` the interior bins 1 .. nK/2 - 1 are doubled: the adjoint of the
-// unnormalised c2r along K.
+// unnormalised c2r along K.  Lines 2f and 2f + 1 share one complex FFT
+// Z = FFT(line 2f + i line 2f + 1): A[t] = (Z[t] + conj Z[-t]) / 2 and
+// B[t] = (Z[t] - conj Z[-t]) / 2i.
 __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, int64_t is, float2 *__restrict__ S,
                                                  BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg,
                                                  int dbl) {
   extern __shared__ float2 smem[];
-  const int nK = M.nK, hK = M.h[M.K];
+  const int nK = M.nK, hK = M.h[M.K], F = (G + 1) / 2;
   float2 *tw = smem;
   float2 *a = tw + nK;
-  float2 *b = a + (size_t)nK * G;
+  float2 *b = a + (size_t)nK * F;
   const int tid = threadIdx.x;
   for (int k = tid; k < nK; k += DT) tw[k] = twg[pl.tw + k];
   const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
   const int64_t l0 = (int64_t)blockIdx.x * G;
   const int valid = (int)min((int64_t)G, nlines - l0);
-  for (int idx = tid; idx < nK * G; idx += DT) {
+  __shared__ LineBases LB;
+  line_bases(M, l0, valid, LB, tid);
+  const int64_t sS = half_stride(M), sC = block_stride(M);
+  __syncthreads();
+  float *af = reinterpret_cast<float *>(a);
+  for (int idx = tid; idx < nK * 2 * F; idx += DT) {
+    // Component g & 1 of FFT g / 2 (slot g = G of an odd G is zero).
     int t, g;
     if (M.K == 0) {
       g = idx / nK;
       t = idx - g * nK;
     } else {
-      t = idx / G;
-      g = idx - t * G;
+      t = idx / (2 * F);
+      g = idx - t * (2 * F);
     }
-    float v = 0.0f;
-    if (g < valid) {
-      int o[3];
-      line_index(M, l0 + g, o);
-      v = in[block_elem(M, o, t) * is];
-    }
-    a[t * G + g] = make_float2(v, 0.0f);
+    af[2 * (t * F + g / 2) + (g & 1)] = g < valid ? in[(LB.c[g] + t * sC) * is] : 0.0f;
   }
   __syncthreads();
-  float2 *res = lds_fft<-1>(a, b, pl, G, tw, tid, DT);
+  float2 *res = lds_fft<-1>(a, b, pl, F, tw, tid, DT);
   for (int idx = tid; idx < hK * G; idx += DT) {
     int t, g;
     if (M.K == 0) {
@@ -390,11 +418,12 @@ __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, 
       g = idx - t * G;
     }
     if (g >= valid) continue;
-    int o[3];
-    line_index(M, l0 + g, o);
-    float2 v = res[t * G + g];
+    const int f = g / 2;
+    const float2 Z = res[t * F + f], Zm = res[((nK - t) % nK) * F + f];
+    float2 v = (g & 1) ? make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x))
+                       : make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
     if (dbl && t >= 1 && t < nK - t) v = cscale(v, 2.0f);
-    S[half_elem(M, o, t)] = v;
+    S[LB.s[g] + t * sS] = v;
   }
 }
 
@@ -726,8 +755,9 @@ struct surfhip_s_plan {
     BlockMap M = block_map(j, K);
     const int nK = M.nK, hK = M.h[K];
     const FftPlan1D &pl = plan_for(nK);
-    int G = std::max(1, std::min(16, 2048 / nK));
-    const size_t lds = sizeof(float2) * ((size_t)nK + (size_t)hK * G + 2 * (size_t)nK * G);
+    const int G = std::max(1, std::min(LINES_MAX, 2048 / nK));
+    const int F = (G + 1) / 2;
+    const size_t lds = sizeof(float2) * ((size_t)nK + (size_t)hK * G + 2 * (size_t)nK * F);
     if (lds > 65536) fail(SURFHIP_ERR_ARG, fmt("axis length %d is too long for the decimated path", nK));
     const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
     const unsigned blocks = (unsigned)((nlines + G - 1) / G);
