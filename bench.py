@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--workload", choices=("ns", "decimated"), default="ns",
+                    help="ns: the NSSurfacelet headline (default); decimated: the decimated Surfacelet "
+                         "(scales=3, direction_levels=[2,2,3], pyramid_ratio=2) on the same volume")
     return ap.parse_args()
 
 
@@ -91,6 +94,90 @@ def cpu_baseline(x_np, budget_s):
                       f"{workers} worker threads, one single-threaded FFT per task), {dt * 1e3:.0f} ms per pair"}
 
 
+def decimated_cpu_baseline(t_gpu, x_np, budget_s):
+    """The FP64 NumPy restatement (oracle/decimated.py) of the decimated pair,
+    single-threaded, on whole pairs until the budget is spent."""
+    from oracle import decimated as OD
+    t = OD.SParams(t_gpu.scales, tuple(t_gpu.direction_levels), pyramid_ratio=t_gpu.pyramid_ratio)
+    x = x_np.astype(np.float64)
+    pairs, t0 = 0, time.perf_counter()
+    while True:
+        c = OD.analyze(t, x)
+        OD.synthesize(t, c, dims=x.shape)
+        pairs += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = (time.perf_counter() - t0) / pairs
+    return {"value": 1.0 / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"{pairs} analyze+synthesize pairs of the same 128^3 workload (FP64 NumPy oracle, "
+                      f"one thread), {dt * 1e3:.0f} ms per pair"}
+
+
+def bench_decimated(args):
+    """Decimated Surfacelet(scales=3, direction_levels=[2,2,3], pyramid_ratio=2)
+    on the 128^3 volume: one step = analyze! + synthesize! with a prebuilt
+    workspace.  Algorithmic bytes per step: the volume and the coefficient
+    vector read or written once per direction, 2 (4 N + 4 ncoef)."""
+    global np
+    import numpy as np
+    import torch
+    from surfhip import decimated as SD
+    from surfhip._arrays import from_numpy
+
+    torch.cuda.set_device(0)
+    t = SD.Surfacelet(scales=3, direction_levels=[2, 2, 3], pyramid_ratio=2)
+    x_np = np.random.default_rng(SEED).standard_normal(DIMS).astype(np.float32)
+    x = from_numpy(x_np, "cuda")
+    ws = SD.make_workspace(t, x)
+    c = SD.similar_coefficients(t, x)
+    y = torch.empty_like(x)
+
+    def step():
+        SD.analyze_(c, t, x, workspace=ws)
+        SD.synthesize_(y, t, c, workspace=ws)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    rel = float(torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x))
+    N = int(np.prod(DIMS))
+    ncoef = int(c.data.numel())
+    bytes_per_step = 2 * (4 * N + 4 * ncoef)
+    achieved = bytes_per_step / (ms * 1e-3) / 1e9
+    result = {
+        "metric": "decimated Surfacelet analyze+synth pairs/s, 128³ FP32, pyramid_ratio 2, 49 subbands",
+        "value": 1e3 / ms,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: i.i.d. N(0,1) Float32 volume, numpy PCG64 seed 20260",
+        "config": {"workload": "Surfacelet(scales=3, direction_levels=[2,2,3], pyramid_ratio=2) "
+                               "analyze!+synthesize!, 128x128x128 Float32, prebuilt workspace",
+                   "dims": list(DIMS), "coefficients": ncoef, "parallelism": "single GPU"},
+        "step_roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS, "bytes_per_step": bytes_per_step},
+        "check": {"reconstruction_rel_l2": rel},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = decimated_cpu_baseline(t, x_np, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -100,6 +187,9 @@ def main():
         print("bench.py measures one GPU: the surfacelet path here runs on a single MI355X "
               "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "decimated":
+        bench_decimated(args)
+        return
     global np
     import numpy as np
     import torch
