@@ -21,6 +21,7 @@
 
 #include "dec_tables.hpp"
 #include "fft.hpp"
+#include "fft_reg.hpp"
 #include "hip_util.hpp"
 
 namespace surfhip {
@@ -28,6 +29,9 @@ namespace surfhip {
 namespace {
 
 constexpr int DT = 256;
+
+// Exchange slot of element e: one pad slot per R1 (bank spread in phase 2).
+template <int R1> __device__ __forceinline__ int xpos_c2c(int e) { return e + e / R1; }
 
 __device__ __forceinline__ int64_t lin3(int i1, int i2, int i3, int m1, int m2) {
   return (int64_t)i1 + (int64_t)m1 * ((int64_t)i2 + (int64_t)m2 * i3);
@@ -96,6 +100,73 @@ __global__ __launch_bounds__(DT) void k_c2c(const float2 *in, float2 *out, int m
       g = idx - t * G;
     }
     if (g < valid) out[base0 + g * sg + t * sa] = res[t * G + g];
+  }
+}
+
+// Power-of-two segments (16..128): the four-step register FFT of fft_reg.hpp
+// with one padded LDS exchange, 256 threads = TP pencils of R2 threads.
+// Pencil p (flattened: segment index, then the other axes) starts at
+// pencil_base(p).  For AXIS 1 the R2 threads of a pencil are consecutive
+// lanes (its segment is contiguous) and results leave through LDS in natural
+// order; for AXIS 2 and 3 consecutive lanes hold consecutive pencils, which are
+// consecutive in memory.
+template <int SEG> constexpr int reg_tp() { return DT / FourStep<SEG>::R2; }
+template <int SEG> constexpr int reg_ps() { return SEG + SEG / FourStep<SEG>::R1 + 1; }
+
+template <int AXIS>
+__device__ __forceinline__ int64_t pencil_base(int64_t p, int m1, int m2, int m3, int seg) {
+  if (AXIS == 1) return p * seg;
+  const int u = (int)(p % m1);
+  const int64_t rest = p / m1;
+  if (AXIS == 2) {
+    const int ns = m2 / seg;
+    return lin3(u, (int)(rest % ns) * seg, (int)(rest / ns), m1, m2);
+  }
+  return lin3(u, (int)(rest % m2), (int)(rest / m2) * seg, m1, m2);
+}
+
+template <int AXIS, int DIR, int SEG>
+__global__ __launch_bounds__(DT) void k_c2c_reg(float2 *A, int m1, int m2, int m3, const float2 *__restrict__ twg) {
+  using F = RegFFT<SEG, DIR>;
+  constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS, TP = reg_tp<SEG>(), PS = reg_ps<SEG>();
+  __shared__ float2 buf[TP * PS];
+  __shared__ float2 tw[SEG];
+  const int tid = threadIdx.x;
+  F::fill_twiddles(tw, twg, tid, DT);
+  const int pp = AXIS == 1 ? tid / R2 : tid % TP, j = AXIS == 1 ? tid % R2 : tid / TP;
+  const int64_t npen = (int64_t)m1 * m2 * m3 / SEG;
+  const int64_t p = (int64_t)blockIdx.x * TP + pp;
+  const bool on = p < npen;
+  const int64_t sa = AXIS == 1 ? 1 : (AXIS == 2 ? (int64_t)m1 : (int64_t)m1 * m2);
+  const int64_t base = on ? pencil_base<AXIS>(p, m1, m2, m3, SEG) : 0;
+  float2 v[R1];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) v[r] = on ? A[base + (j + R2 * r) * sa] : make_float2(0.0f, 0.0f);
+  __syncthreads();  // twiddles
+  F::phase1(v, j, tw);
+  float2 *pb = buf + pp * PS;
+#pragma unroll
+  for (int s = 0; s < R1; ++s) pb[xpos_c2c<R1>(s * R2 + j)] = v[s];
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < R1; ++e) v[e] = pb[xpos_c2c<R1>(j * R1 + e)];
+  F::phase2(v);
+  if (AXIS == 1) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < ROWS; ++u)
+#pragma unroll
+      for (int q = 0; q < R2; ++q) pb[(j * ROWS + u) + R1 * q] = v[u * R2 + q];
+    __syncthreads();
+    // The block's TP segments are one contiguous run of TP SEG elements.
+    const int64_t p0 = (int64_t)blockIdx.x * TP;
+    const int nvalid = (int)min((int64_t)TP, npen - p0) * SEG;
+    for (int e = tid; e < nvalid; e += DT) A[p0 * SEG + e] = buf[(e / SEG) * PS + e % SEG];
+  } else if (on) {
+#pragma unroll
+    for (int u = 0; u < ROWS; ++u)
+#pragma unroll
+      for (int q = 0; q < R2; ++q) A[base + (int64_t)((j * ROWS + u) + R1 * q) * sa] = v[u * R2 + q];
   }
 }
 
@@ -642,6 +713,27 @@ struct surfhip_s_plan {
 
   void launch_c2c(int axis, int dir, float2 *A, Dims3 m, int seg) {
     const FftPlan1D &pl = plan_for(seg);
+    if (seg == 16 || seg == 32 || seg == 64 || seg == 128) {
+      const int64_t npen = (int64_t)m[0] * m[1] * m[2] / seg;
+      const float2 *twp = tw.p + pl.tw;
+#define SH_C2CR(AX, D, SG)                                                                                       \
+  hipLaunchKernelGGL((k_c2c_reg<AX, D, SG>), dim3((unsigned)((npen + reg_tp<SG>() - 1) / reg_tp<SG>())), dim3(DT), 0, \
+                     ls, A, m[0], m[1], m[2], twp)
+#define SH_C2CR_SEG(AX, D)                   \
+  switch (seg) {                             \
+    case 16: SH_C2CR(AX, D, 16); break;      \
+    case 32: SH_C2CR(AX, D, 32); break;      \
+    case 64: SH_C2CR(AX, D, 64); break;      \
+    default: SH_C2CR(AX, D, 128); break;     \
+  }
+      if (axis == 0) { if (dir < 0) { SH_C2CR_SEG(1, -1) } else { SH_C2CR_SEG(1, 1) } }
+      else if (axis == 1) { if (dir < 0) { SH_C2CR_SEG(2, -1) } else { SH_C2CR_SEG(2, 1) } }
+      else { if (dir < 0) { SH_C2CR_SEG(3, -1) } else { SH_C2CR_SEG(3, 1) } }
+#undef SH_C2CR_SEG
+#undef SH_C2CR
+      SH_LAUNCHED();
+      return;
+    }
     int G = std::max(1, std::min(16, 4096 / seg));
     const size_t lds = sizeof(float2) * ((size_t)seg + 2 * (size_t)seg * G);
     if (lds > 65536) fail(SURFHIP_ERR_ARG, fmt("axis length %d is too long for the decimated path", seg));
