@@ -510,6 +510,7 @@ __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, 
 // place on S.
 struct IrcGroup {
   const float2 *f0[8], *f1[8];  // per stage in stage order: m1 levels 1..L1, then m2 levels 1..L2
+  int sc[8], sk[8];             // value (k, c) of a plane at c sc + k sk
 };
 
 template <int L1, int L2>
@@ -561,6 +562,8 @@ __global__ __launch_bounds__(DT) void k_irc_group(const float2 *__restrict__ X, 
       const bool on1 = s < L1;
       const int bit = on1 ? L1 - 1 - s : T - 1 - (s - L1);
       const float2 *f0 = F.f0[s], *f1 = F.f1[s];
+      const int sc = F.sc[s], sk = F.sk[s];
+      auto at = [&](const float2 *f, int c) { return f[(int64_t)c * sc + (int64_t)iK * sk]; };
 #pragma unroll
       for (int gA = 0; gA < NG; ++gA) {
         if (gA & (1 << bit)) continue;
@@ -568,8 +571,8 @@ __global__ __launch_bounds__(DT) void k_irc_group(const float2 *__restrict__ X, 
         const int cA = on1 ? r1 + (gA & ((1 << L1) - 1)) * B1 : r2 + (gA >> L1) * B2;
         const int cB = on1 ? r1 + (gB & ((1 << L1) - 1)) * B1 : r2 + (gB >> L1) * B2;
         const float2 A = v[gA], B = v[gB];
-        const float2 f0A = f0[iK + (int64_t)nKh * cA], f0B = f0[iK + (int64_t)nKh * cB];
-        const float2 f1A = f1[iK + (int64_t)nKh * cA], f1B = f1[iK + (int64_t)nKh * cB];
+        const float2 f0A = at(f0, cA), f0B = at(f0, cB);
+        const float2 f1A = at(f1, cA), f1B = at(f1, cB);
         if (mode == 0) {
           v[gA] = cadd(cmul(f0A, A), cmul(f0B, B));
           v[gB] = cadd(cmul(f1A, A), cmul(f1B, B));
@@ -701,6 +704,7 @@ struct surfhip_s_plan {
   std::vector<std::unique_ptr<DecScaleDev>> sc;
   DevBuf<float> masks;
   DevBuf<float2> filt, tw;
+  DevBuf<float2> filt_t;  // transposed planes of the stages along axis 0 of hourglasses 2 and 3
   std::vector<DevBuf<float2>> spectra;  // [j] the spectrum entering level j; [J] the coarse lowpass
   DevBuf<float2> Sbuf[3];
   std::vector<std::pair<int, FftPlan1D>> plans;
@@ -782,8 +786,14 @@ struct surfhip_s_plan {
     const auto &stg = s.stages[K];
     if ((int)stg.size() != L1 + L2) fail(SURFHIP_ERR_DEVICE, "internal: IRC stage count");
     for (int q = 0; q < L1 + L2; ++q) {
-      F.f0[q] = filt.p + stg[q].off[mode == 1 ? 2 : 0];
-      F.f1[q] = filt.p + stg[q].off[mode == 1 ? 3 : 1];
+      const int a = mode == 1 ? 2 : 0;
+      const bool tr = stg[q].offT[a] >= 0;
+      const float2 *base = tr ? filt_t.p : filt.p;
+      const int *off = tr ? stg[q].offT : stg[q].off;
+      F.f0[q] = base + off[a];
+      F.f1[q] = base + off[a + 1];
+      F.sc[q] = tr ? 1 : hd[K];
+      F.sk[q] = tr ? hd[stg[q].m] : 1;
     }
     const int64_t ng = (int64_t)hd[K] * (hd[m1] >> L1) * (hd[m2] >> L2);
     const dim3 grid(grid_for(ng)), block(DT);
@@ -1108,6 +1118,26 @@ int surfhip_s_plan_create(surfhip_s_plan **out, const surfhip_s_params *p, int32
     P->masks.upload(T.masks);
     if (T.filt.empty()) P->T.filt.push_back(make_float2(0, 0));
     P->filt.upload(P->T.filt);
+    {
+      // In k_irc_group the lanes of hourglass K != 0 run along axis 0, so a
+      // stage along axis 0 reads its planes across c: keep those transposed.
+      std::vector<float2> ft;
+      for (DecScaleT &s : P->T.scales)
+        for (int K = 1; K < 3; ++K) {
+          const Dims3 hd = half_dims(s.n, K);
+          for (IRCStageT &st : s.stages[K]) {
+            if (st.m != 0) continue;
+            const int nKh = hd[K], nm = hd[0];
+            for (int f = 0; f < 4; ++f) {
+              st.offT[f] = (int)ft.size();
+              for (int k = 0; k < nKh; ++k)
+                for (int c = 0; c < nm; ++c) ft.push_back(P->T.filt[(size_t)st.off[f] + k + (size_t)nKh * c]);
+            }
+          }
+        }
+      if (ft.empty()) ft.push_back(make_float2(0, 0));
+      P->filt_t.upload(ft);
+    }
     // Pyramid tables.
     for (const PyrLevel &pl : T.pyr) {
       auto L = std::make_unique<DecLevelDev>();

@@ -33,6 +33,7 @@ struct IRCStageT {
   int m;   // axis decimated (0-based)
   int b;   // block size after the stage
   int off[4];  // ana0, ana1, syn0, syn1 plane offsets (complex pool)
+  int offT[4] = {-1, -1, -1, -1};  // the same planes transposed ([c + n_m k]), when the device keeps them
 };
 
 struct DecScaleT {
