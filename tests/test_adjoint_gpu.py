@@ -63,6 +63,7 @@ def test_ns_adjoint_identities(dims, tight):
 @pytest.mark.parametrize("dims,mode,J,L,bo", [
     ((32, 32, 32), 2, 2, [1, 2], 4),
     ((48, 24, 36), 1.5, 1, [2], 8),
+    ((32, 32, 32), 1, 1, [5], 6),  # per-stage IRC kernel (3 + 2 stages)
 ])
 def test_decimated_adjoint_matches_oracle(dims, mode, J, L, bo):
     dev = _dev()

@@ -70,6 +70,9 @@ CASES = [
     ((64, 32, 32), 2, 2, [1, 3], 18),
     ((32, 32, 32), 2, 1, [np.array([[-1, 2, 0], [1, -1, 1], [0, 3, -1]])], 6),
     ((24, 24, 24), 1.5, 1, [0], 12),
+    # level 5 splits into 3 + 2 IRC stages: groups of 32 elements take the
+    # per-stage kernel instead of the grouped one
+    ((64, 64, 64), 2, 1, [5], 6),
 ]
 
 
