@@ -16,7 +16,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <memory>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "dec_tables.hpp"
@@ -674,7 +677,107 @@ struct surfhip_s_plan {
   hipStream_t ls = nullptr;
   hipStream_t kst[3] = {nullptr, nullptr, nullptr};
   hipEvent_t kfork = nullptr, kjoin[3] = {nullptr, nullptr, nullptr};
+
+  // Launch recording.  A pair issues about a hundred launches, which costs
+  // the host about as long as the GPU takes to run them.  So a call that
+  // repeats the previous call's data pointers is recorded once as a hipGraph
+  // (kernel nodes added explicitly, no stream capture) and replayed with one
+  // hipGraphLaunch.  While `rec` is set, go() adds a node after the previous
+  // nodes of lane `lane` (0 = `stream`, K + 1 = hourglass K's stream), and
+  // fork_k/join_k add the edges their events stand for.
+  struct Rec {
+    hipGraph_t g = nullptr;
+    std::array<std::vector<hipGraphNode_t>, 4> deps;
+    std::vector<std::shared_ptr<void>> keep;  // kernel arguments of the nodes
+  };
+  Rec *rec = nullptr;
+  int lane = 0;
+  void use_lane(int L) {
+    lane = L;
+    ls = L == 0 ? stream : kst[L - 1];
+  }
+  template <class Tup, size_t... I> static void arg_ptrs(Tup &t, void **kp, std::index_sequence<I...>) {
+    ((kp[I] = (void *)&std::get<I>(t)), ...);
+  }
+  template <typename... P, typename... A>
+  void go(void (*k)(P...), dim3 grid, dim3 block, size_t lds, A... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    auto t = std::make_shared<std::tuple<P...>>(static_cast<P>(a)...);
+    void *kp[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    arg_ptrs(*t, kp, std::index_sequence_for<P...>{});
+    if (!rec) {
+      SH_HIP(hipLaunchKernel((const void *)k, grid, block, kp, lds, ls));
+      return;
+    }
+    hipKernelNodeParams p{};
+    p.func = (void *)k;
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = (unsigned)lds;
+    p.kernelParams = kp;
+    p.extra = nullptr;
+    hipGraphNode_t n;
+    std::vector<hipGraphNode_t> &d = rec->deps[lane];
+    SH_HIP(hipGraphAddKernelNode(&n, rec->g, d.empty() ? nullptr : d.data(), d.size(), &p));
+    d.assign(1, n);
+    rec->keep.push_back(t);
+  }
+  // One recorded call per direction (0 analyze, 1 synthesize, 2 adjoint).
+  struct Cached {
+    const void *in = nullptr, *out = nullptr;
+    int seen = 0;
+    Rec rec;
+    hipGraphExec_t exec = nullptr;
+  };
+  Cached cache[3];
+  static void drop(Cached &c) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.rec.g) (void)hipGraphDestroy(c.rec.g);
+    c.exec = nullptr;
+    c.rec = Rec{};
+  }
+  // body() issues the call's launches.  Direct launches unless the call
+  // repeats the previous one's pointers; a stream being captured by the
+  // caller always gets direct launches.
+  template <class F> void run(int kind, const void *in, const void *out, F body) {
+    Cached &c = cache[kind];
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &st) != hipSuccess) {
+      (void)hipGetLastError();
+      st = hipStreamCaptureStatusActive;
+    }
+    if (st != hipStreamCaptureStatusNone) {
+      body();
+      return;
+    }
+    if (c.in != in || c.out != out) {
+      drop(c);
+      c.in = in;
+      c.out = out;
+      c.seen = 0;
+    }
+    if (!c.exec && ++c.seen >= 2) {
+      SH_HIP(hipGraphCreate(&c.rec.g, 0));
+      rec = &c.rec;
+      try {
+        body();
+      } catch (...) {
+        rec = nullptr;
+        drop(c);
+        throw;
+      }
+      rec = nullptr;
+      SH_HIP(hipGraphInstantiate(&c.exec, c.rec.g, nullptr, nullptr, 0));
+    }
+    if (c.exec) {
+      SH_HIP(hipGraphLaunch(c.exec, stream));
+      return;
+    }
+    body();
+  }
+
   ~surfhip_s_plan() {
+    for (Cached &c : cache) drop(c);
     for (int K = 0; K < 3; ++K) {
       if (kst[K]) (void)hipStreamDestroy(kst[K]);
       if (kjoin[K]) (void)hipEventDestroy(kjoin[K]);
@@ -690,11 +793,31 @@ struct surfhip_s_plan {
   }
   // Later work of the hourglass streams follows everything already on `stream`.
   void fork_k() {
+    if (rec) {
+      // A lane keeps the order of its own earlier nodes, as a stream does.
+      for (int L = 1; L <= 3; ++L) {
+        std::vector<hipGraphNode_t> &d = rec->deps[L];
+        d.insert(d.end(), rec->deps[0].begin(), rec->deps[0].end());
+        std::sort(d.begin(), d.end());
+        d.erase(std::unique(d.begin(), d.end()), d.end());
+      }
+      return;
+    }
     SH_HIP(hipEventRecord(kfork, stream));
     for (int K = 0; K < 3; ++K) SH_HIP(hipStreamWaitEvent(kst[K], kfork, 0));
   }
   // Later work of `stream` follows everything already on the hourglass streams.
   void join_k() {
+    if (rec) {
+      std::vector<hipGraphNode_t> &d0 = rec->deps[0];
+      for (int L = 1; L <= 3; ++L) {
+        d0.insert(d0.end(), rec->deps[L].begin(), rec->deps[L].end());
+        rec->deps[L].clear();
+      }
+      std::sort(d0.begin(), d0.end());
+      d0.erase(std::unique(d0.begin(), d0.end()), d0.end());
+      return;
+    }
     for (int K = 0; K < 3; ++K) {
       SH_HIP(hipEventRecord(kjoin[K], kst[K]));
       SH_HIP(hipStreamWaitEvent(stream, kjoin[K], 0));
@@ -721,8 +844,7 @@ struct surfhip_s_plan {
       const int64_t npen = (int64_t)m[0] * m[1] * m[2] / seg;
       const float2 *twp = tw.p + pl.tw;
 #define SH_C2CR(AX, D, SG)                                                                                       \
-  hipLaunchKernelGGL((k_c2c_reg<AX, D, SG>), dim3((unsigned)((npen + reg_tp<SG>() - 1) / reg_tp<SG>())), dim3(DT), 0, \
-                     ls, A, m[0], m[1], m[2], twp)
+  go((k_c2c_reg<AX, D, SG>), dim3((unsigned)((npen + reg_tp<SG>() - 1) / reg_tp<SG>())), dim3(DT), 0, A, m[0], m[1], m[2], twp)
 #define SH_C2CR_SEG(AX, D)                   \
   switch (seg) {                             \
     case 16: SH_C2CR(AX, D, 16); break;      \
@@ -735,7 +857,6 @@ struct surfhip_s_plan {
       else { if (dir < 0) { SH_C2CR_SEG(3, -1) } else { SH_C2CR_SEG(3, 1) } }
 #undef SH_C2CR_SEG
 #undef SH_C2CR
-      SH_LAUNCHED();
       return;
     }
     int G = std::max(1, std::min(16, 4096 / seg));
@@ -750,19 +871,17 @@ struct surfhip_s_plan {
       blocks = axis == 1 ? ub * (m[1] / seg) * m[2] : ub * m[1] * (m[2] / seg);
     }
 #define SH_C2C(AX, D) \
-  hipLaunchKernelGGL((k_c2c<AX, D>), dim3((unsigned)blocks), dim3(DT), lds, ls, A, A, m[0], m[1], m[2], seg, G, pl, tw.p)
+  go((k_c2c<AX, D>), dim3((unsigned)blocks), dim3(DT), lds, A, A, m[0], m[1], m[2], seg, G, pl, tw.p)
     if (axis == 0) { if (dir < 0) SH_C2C(1, -1); else SH_C2C(1, 1); }
     else if (axis == 1) { if (dir < 0) SH_C2C(2, -1); else SH_C2C(2, 1); }
     else { if (dir < 0) SH_C2C(3, -1); else SH_C2C(3, 1); }
 #undef SH_C2C
-    SH_LAUNCHED();
   }
   void fft3(int dir, float2 *A, Dims3 m) {
     for (int ax = 0; ax < 3; ++ax) launch_c2c(ax, dir, A, m, m[ax]);
   }
   void zero(float2 *p, int64_t n) {
-    hipLaunchKernelGGL(k_dzero, dim3(grid_for(n)), dim3(DT), 0, ls, p, n);
-    SH_LAUNCHED();
+    go(k_dzero, dim3(grid_for(n)), dim3(DT), 0, p, n);
   }
   static void transverse(int K, int &m1, int &m2) {
     m1 = K == 0 ? 1 : 0;
@@ -798,7 +917,7 @@ struct surfhip_s_plan {
     const int64_t ng = (int64_t)hd[K] * (hd[m1] >> L1) * (hd[m2] >> L2);
     const dim3 grid(grid_for(ng)), block(DT);
 #define SH_IRCG(A, B)                                                                                                \
-  hipLaunchKernelGGL((k_irc_group<A, B>), grid, block, 0, ls, X, s.n[0], s.n[1], V, scale, S, hd[0], hd[1], hd[2], K, \
+  go((k_irc_group<A, B>), grid, block, 0, X, s.n[0], s.n[1], V, scale, S, hd[0], hd[1], hd[2], K, \
                      m1, m2, F, mode)
     switch (L1 * 5 + L2) {
       case 0: SH_IRCG(0, 0); break;
@@ -819,7 +938,6 @@ struct surfhip_s_plan {
       default: fail(SURFHIP_ERR_DEVICE, "internal: IRC group size");
     }
 #undef SH_IRCG
-    SH_LAUNCHED();
   }
   // X of scale j = the hourglass spectra summed back (with their Hermitian
   // mirrors when `mirror`), each scaled by c[K] and its ritf mask.
@@ -832,8 +950,7 @@ struct surfhip_s_plan {
       H.c[K] = c[K];
     }
     const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
-    hipLaunchKernelGGL(k_hermitian_sum, dim3(grid_for(NX)), dim3(DT), 0, ls, X, s.n[0], s.n[1], s.n[2], H, mirror);
-    SH_LAUNCHED();
+    go(k_hermitian_sum, dim3(grid_for(NX)), dim3(DT), 0, X, s.n[0], s.n[1], s.n[2], H, mirror);
   }
   BlockMap block_map(int j, int K) const {
     const DecScaleT &s = T.scales[j];
@@ -864,34 +981,31 @@ struct surfhip_s_plan {
     const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
     const unsigned blocks = (unsigned)((nlines + G - 1) / G);
     if (c2r)
-      hipLaunchKernelGGL(k_c2r_lines, dim3(blocks), dim3(DT), lds, ls, S, coef, stride, M, G, pl, tw.p);
+      go(k_c2r_lines, dim3(blocks), dim3(DT), lds, S, coef, stride, M, G, pl, tw.p);
     else
-      hipLaunchKernelGGL(k_r2c_lines, dim3(blocks), dim3(DT), lds, ls, (const float *)coef, stride, S, M, G, pl,
+      go(k_r2c_lines, dim3(blocks), dim3(DT), lds, (const float *)coef, stride, S, M, G, pl,
                          tw.p, dbl);
-    SH_LAUNCHED();
   }
 
   void analyze_channel(const float *x, float *coef) {
-    ls = stream;
+    use_lane(0);
     const int64_t os = nch;
     Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
     float2 *X = spectra[0].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, ls, x, (int64_t)nch, X, N, 1.0f);
-    SH_LAUNCHED();
+    go(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, x, (int64_t)nch, X, N, 1.0f);
     fft3(-1, X, n);
     for (int j = 0; j < T.J; ++j) {
       const PyrLevel &pl = T.pyr[j];
       float2 *low = spectra[j + 1].p;
       zero(low, (int64_t)pl.nlow[0] * pl.nlow[1] * pl.nlow[2]);
       const PyrDev &pd = lev[j]->pd;
-      hipLaunchKernelGGL(k_pyr_split, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls,
+      go(k_pyr_split, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0,
                          spectra[j].p, low, pd);
-      SH_LAUNCHED();
       const DecScaleT &s = T.scales[j];
       fork_k();
       for (int K = 0; K < 3; ++K) {
-        ls = kst[K];
+        use_lane(K + 1);
         Dims3 hd = half_dims(s.n, K);
         Dims3 sb = subband_size(s, K);
         const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
@@ -900,13 +1014,11 @@ struct surfhip_s_plan {
         if (grouped(s, K)) {
           irc_group(s, K, 0, spectra[j].p, masks.p + s.mask_off[K], scale, S);
         } else {
-          hipLaunchKernelGGL(k_masked, dim3(grid_for(HN)), dim3(DT), 0, ls, spectra[j].p, s.n[0], s.n[1], S,
+          go(k_masked, dim3(grid_for(HN)), dim3(DT), 0, spectra[j].p, s.n[0], s.n[1], S,
                              masks.p + s.mask_off[K], hd[0], hd[1], hd[2], scale);
-          SH_LAUNCHED();
           for (const IRCStageT &st : s.stages[K]) {
-            hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m, st.b,
+            go(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, S, hd[0], hd[1], hd[2], K, st.m, st.b,
                                filt.p + st.off[0], filt.p + st.off[1], 0);
-            SH_LAUNCHED();
           }
         }
         Dims3 counts, bsize;
@@ -915,34 +1027,32 @@ struct surfhip_s_plan {
           if (d != K) launch_c2c(d, 1, S, hd, bsize[d]);
         lines(true, j, K, S, coef, os);
       }
-      ls = stream;
+      use_lane(0);
     }
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
     fft3(1, C, nl);
-    hipLaunchKernelGGL(k_real_out, dim3(grid_for(NL)), dim3(DT), 0, ls, C, coef + T.offsets[0] * nch, os,
+    go(k_real_out, dim3(grid_for(NL)), dim3(DT), 0, C, coef + T.offsets[0] * nch, os,
                        (float)(1.0 / (double)NL), NL);
-    SH_LAUNCHED();
     join_k();
   }
 
   void synthesize_channel(const float *coef, float *x) {
-    ls = stream;
+    use_lane(0);
     const int64_t is = nch;
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, ls, coef + T.offsets[0] * nch, is, C, NL,
+    go(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, coef + T.offsets[0] * nch, is, C, NL,
                        1.0f);
-    SH_LAUNCHED();
     fft3(-1, C, nl);
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
       fork_k();
       for (int K = 0; K < 3; ++K) {
-        ls = kst[K];
+        use_lane(K + 1);
         Dims3 hd = half_dims(s.n, K);
         float2 *S = Sbuf[K].p;
         lines(false, j, K, S, const_cast<float *>(coef), is);
@@ -956,49 +1066,45 @@ struct surfhip_s_plan {
         } else {
           for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
             const IRCStageT &st = s.stages[K][q];
-            hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m,
+            go(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, S, hd[0], hd[1], hd[2], K, st.m,
                                st.b, filt.p + st.off[2], filt.p + st.off[3], 1);
-            SH_LAUNCHED();
           }
         }
       }
-      ls = stream;
+      use_lane(0);
       join_k();
       {
         const float c[3] = {(float)(1.0 / 3.0), (float)(1.0 / 3.0), (float)(1.0 / 3.0)};
         hermitian_sum(j, X, c, 1);
       }
       const PyrDev &pd = lev[j]->pd;
-      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls, X,
+      go(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, X,
                          spectra[j + 1].p, pd, pd.D15);
-      SH_LAUNCHED();
     }
     const Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
     fft3(1, spectra[0].p, n);
-    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, ls, spectra[0].p, x, (int64_t)nch,
+    go(k_real_out, dim3(grid_for(N)), dim3(DT), 0, spectra[0].p, x, (int64_t)nch,
                        (float)(1.0 / (double)N), N);
-    SH_LAUNCHED();
   }
 
   // x = A^T c, the adjoint of the analysis (not its inverse: the transform
   // is a frame but not a tight one, src/transforms/surfacelet.jl:24-26).
   void adjoint_channel(const float *coef, float *x) {
-    ls = stream;
+    use_lane(0);
     const int64_t is = nch;
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    hipLaunchKernelGGL(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, ls, coef + T.offsets[0] * nch, is, C, NL,
+    go(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, coef + T.offsets[0] * nch, is, C, NL,
                        (float)(1.0 / (double)NL));
-    SH_LAUNCHED();
     fft3(-1, C, nl);
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
       fork_k();
       for (int K = 0; K < 3; ++K) {
-        ls = kst[K];
+        use_lane(K + 1);
         Dims3 hd = half_dims(s.n, K);
         float2 *S = Sbuf[K].p;
         lines(false, j, K, S, const_cast<float *>(coef), is, 1);
@@ -1012,13 +1118,12 @@ struct surfhip_s_plan {
         } else {
           for (int q = (int)s.stages[K].size() - 1; q >= 0; --q) {
             const IRCStageT &st = s.stages[K][q];
-            hipLaunchKernelGGL(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, ls, S, hd[0], hd[1], hd[2], K, st.m,
+            go(k_irc, dim3(grid_for(HN / 2)), dim3(DT), 0, S, hd[0], hd[1], hd[2], K, st.m,
                                st.b, filt.p + st.off[0], filt.p + st.off[1], 2);
-            SH_LAUNCHED();
           }
         }
       }
-      ls = stream;
+      use_lane(0);
       join_k();
       {
         float c[3];
@@ -1029,15 +1134,13 @@ struct surfhip_s_plan {
         hermitian_sum(j, X, c, 0);
       }
       const PyrDev &pd = lev[j]->pd;
-      hipLaunchKernelGGL(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, ls, X,
+      go(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, X,
                          spectra[j + 1].p, pd, 1.0f / pd.D15);
-      SH_LAUNCHED();
     }
     const Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
     fft3(1, spectra[0].p, n);
-    hipLaunchKernelGGL(k_real_out, dim3(grid_for(N)), dim3(DT), 0, ls, spectra[0].p, x, (int64_t)nch, 1.0f, N);
-    SH_LAUNCHED();
+    go(k_real_out, dim3(grid_for(N)), dim3(DT), 0, spectra[0].p, x, (int64_t)nch, 1.0f, N);
   }
 };
 
@@ -1221,21 +1324,27 @@ int surfhip_s_set_stream(surfhip_s_plan *plan, void *stream) {
 int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
-    for (int ch = 0; ch < plan->nch; ++ch) plan->analyze_channel((const float *)x + ch, (float *)coeffs + ch);
+    plan->run(0, x, coeffs, [&] {
+      for (int ch = 0; ch < plan->nch; ++ch) plan->analyze_channel((const float *)x + ch, (float *)coeffs + ch);
+    });
   });
 }
 
 int surfhip_s_adjoint(surfhip_s_plan *plan, const void *coeffs, void *x) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
-    for (int ch = 0; ch < plan->nch; ++ch) plan->adjoint_channel((const float *)coeffs + ch, (float *)x + ch);
+    plan->run(2, coeffs, x, [&] {
+      for (int ch = 0; ch < plan->nch; ++ch) plan->adjoint_channel((const float *)coeffs + ch, (float *)x + ch);
+    });
   });
 }
 
 int surfhip_s_synthesize(surfhip_s_plan *plan, const void *coeffs, void *x) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
-    for (int ch = 0; ch < plan->nch; ++ch) plan->synthesize_channel((const float *)coeffs + ch, (float *)x + ch);
+    plan->run(1, coeffs, x, [&] {
+      for (int ch = 0; ch < plan->nch; ++ch) plan->synthesize_channel((const float *)coeffs + ch, (float *)x + ch);
+    });
   });
 }
 

@@ -159,3 +159,48 @@ def test_surfacelet_128_phantom_config():
     assert _rel(to_numpy(c.data), flat_ref) <= RTOL
     xr = to_numpy(S.synthesize(t, c, workspace=ws))
     assert _rel(xr, x) <= RTOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims,J,L", [((128, 128, 128), 3, [2, 2, 3]), ((64, 64, 64), 3, [2, 2, 3]),
+                                     ((48, 24, 36), 2, [1, 2])])
+def test_surfacelet_graph_replays_match_direct(dims, J, L):
+    # A call that repeats the previous call's pointers is recorded as a graph
+    # and replayed: every replay, with eager work and synchronisation between
+    # them, equals the first (directly launched) call bit for bit.  Calls that
+    # alternate between two outputs keep launching directly.
+    dev = _dev()
+    print(f"case graph replay dims={dims}", flush=True)
+    rng = np.random.default_rng(310)
+    t = S.Surfacelet(scales=J, direction_levels=L, pyramid_ratio=2 if dims[0] in (64, 128) else 1.5)
+    x = from_numpy(rng.standard_normal(dims).astype(np.float32), dev)
+    ws = S.make_workspace(t, x)
+    c = S.similar_coefficients(t, x)
+    y = torch.empty_like(x)
+    S.analyze_(c, t, x, workspace=ws)
+    S.synthesize_(y, t, c, workspace=ws)
+    c0, y0 = c.data.clone(), y.clone()
+    assert _rel(to_numpy(y0), to_numpy(x)) <= RTOL
+    for k in range(4):
+        noise = torch.randn(1 << 20, device=dev)  # eager work between replays
+        c.data.fill_(float("nan"))
+        y.fill_(float("nan"))
+        S.analyze_(c, t, x, workspace=ws)
+        S.synthesize_(y, t, c, workspace=ws)
+        torch.cuda.synchronize()
+        assert float(noise.abs().sum()) > 0
+        assert torch.equal(c.data, c0), k
+        assert torch.equal(y, y0), k
+    # alternating outputs: direct launches, same results
+    c2 = S.similar_coefficients(t, x)
+    for k in range(3):
+        S.analyze_(c2 if k % 2 else c, t, x, workspace=ws)
+    torch.cuda.synchronize()
+    assert torch.equal(c2.data, c0) and torch.equal(c.data, c0)
+    # the adjoint's replays match its first call
+    a0 = S.adjoint(t, c, workspace=ws).clone()
+    a = torch.empty_like(x)
+    for k in range(3):
+        a.copy_(S.adjoint(t, c, workspace=ws))
+        torch.cuda.synchronize()
+        assert torch.equal(a, a0), k
