@@ -128,8 +128,19 @@ __device__ __forceinline__ int64_t pencil_base(int64_t p, int m1, int m2, int m3
   return lin3(u, (int)(rest % m2), (int)(rest / m2) * seg, m1, m2);
 }
 
-template <int AXIS, int DIR, int SEG>
-__global__ __launch_bounds__(DT) void k_c2c_reg(float2 *A, int m1, int m2, int m3, const float2 *__restrict__ twg) {
+// Optional real end of a volume transform: IO 1 reads the real input
+// rin[i stride] scale (imaginary part 0) instead of A; IO 2 writes
+// rout[i stride] = Re(result) scale instead of A (i = linear element index).
+struct RealIO {
+  const float *in;
+  float *out;
+  int64_t stride;
+  float scale;
+};
+
+template <int AXIS, int DIR, int SEG, int IO = 0>
+__global__ __launch_bounds__(DT) void k_c2c_reg(float2 *A, int m1, int m2, int m3, const float2 *__restrict__ twg,
+                                               RealIO rio) {
   using F = RegFFT<SEG, DIR>;
   constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS, TP = reg_tp<SEG>(), PS = reg_ps<SEG>();
   __shared__ float2 buf[TP * PS];
@@ -144,7 +155,11 @@ __global__ __launch_bounds__(DT) void k_c2c_reg(float2 *A, int m1, int m2, int m
   const int64_t base = on ? pencil_base<AXIS>(p, m1, m2, m3, SEG) : 0;
   float2 v[R1];
 #pragma unroll
-  for (int r = 0; r < R1; ++r) v[r] = on ? A[base + (j + R2 * r) * sa] : make_float2(0.0f, 0.0f);
+  for (int r = 0; r < R1; ++r) {
+    const int64_t i = base + (j + R2 * r) * sa;
+    if (IO == 1) v[r] = make_float2(on ? rio.in[i * rio.stride] * rio.scale : 0.0f, 0.0f);
+    else v[r] = on ? A[i] : make_float2(0.0f, 0.0f);
+  }
   __syncthreads();  // twiddles
   F::phase1(v, j, tw);
   float2 *pb = buf + pp * PS;
@@ -164,12 +179,20 @@ __global__ __launch_bounds__(DT) void k_c2c_reg(float2 *A, int m1, int m2, int m
     // The block's TP segments are one contiguous run of TP SEG elements.
     const int64_t p0 = (int64_t)blockIdx.x * TP;
     const int nvalid = (int)min((int64_t)TP, npen - p0) * SEG;
-    for (int e = tid; e < nvalid; e += DT) A[p0 * SEG + e] = buf[(e / SEG) * PS + e % SEG];
+    for (int e = tid; e < nvalid; e += DT) {
+      const float2 z = buf[(e / SEG) * PS + e % SEG];
+      if (IO == 2) rio.out[(p0 * SEG + e) * rio.stride] = z.x * rio.scale;
+      else A[p0 * SEG + e] = z;
+    }
   } else if (on) {
 #pragma unroll
     for (int u = 0; u < ROWS; ++u)
 #pragma unroll
-      for (int q = 0; q < R2; ++q) A[base + (int64_t)((j * ROWS + u) + R1 * q) * sa] = v[u * R2 + q];
+      for (int q = 0; q < R2; ++q) {
+        const int64_t i = base + (int64_t)((j * ROWS + u) + R1 * q) * sa;
+        if (IO == 2) rio.out[i * rio.stride] = v[u * R2 + q].x * rio.scale;
+        else A[i] = v[u * R2 + q];
+      }
   }
 }
 
@@ -838,13 +861,35 @@ struct surfhip_s_plan {
     fail(SURFHIP_ERR_DEVICE, fmt("no FFT plan for length %d", n));
   }
 
-  void launch_c2c(int axis, int dir, float2 *A, Dims3 m, int seg) {
+  static bool reg_len(int n) { return n == 16 || n == 32 || n == 64 || n == 128; }
+  // io: 0 complex A in and out; 1 (axis 0, forward) real input rio; 2 (axis 2,
+  // inverse) real output rio.  io != 0 needs a register-FFT length.
+  void launch_c2c(int axis, int dir, float2 *A, Dims3 m, int seg, int io = 0, RealIO rio = RealIO{}) {
     const FftPlan1D &pl = plan_for(seg);
-    if (seg == 16 || seg == 32 || seg == 64 || seg == 128) {
+    if (io == 1 && !(axis == 0 && dir < 0 && reg_len(seg))) fail(SURFHIP_ERR_DEVICE, "internal: real-input FFT pass");
+    if (io == 2 && !(axis == 2 && dir > 0 && reg_len(seg))) fail(SURFHIP_ERR_DEVICE, "internal: real-output FFT pass");
+    if (reg_len(seg)) {
       const int64_t npen = (int64_t)m[0] * m[1] * m[2] / seg;
       const float2 *twp = tw.p + pl.tw;
+      if (io) {
+#define SH_C2CIO(AX, D, SG, IO)                                                                                 \
+  go((k_c2c_reg<AX, D, SG, IO>), dim3((unsigned)((npen + reg_tp<SG>() - 1) / reg_tp<SG>())), dim3(DT), 0, A, m[0], \
+     m[1], m[2], twp, rio)
+#define SH_C2CIO_SEG(AX, D, IO)                 \
+  switch (seg) {                                \
+    case 16: SH_C2CIO(AX, D, 16, IO); break;    \
+    case 32: SH_C2CIO(AX, D, 32, IO); break;    \
+    case 64: SH_C2CIO(AX, D, 64, IO); break;    \
+    default: SH_C2CIO(AX, D, 128, IO); break;   \
+  }
+        if (io == 1) { SH_C2CIO_SEG(1, -1, 1) } else { SH_C2CIO_SEG(3, 1, 2) }
+#undef SH_C2CIO_SEG
+#undef SH_C2CIO
+        return;
+      }
 #define SH_C2CR(AX, D, SG)                                                                                       \
-  go((k_c2c_reg<AX, D, SG>), dim3((unsigned)((npen + reg_tp<SG>() - 1) / reg_tp<SG>())), dim3(DT), 0, A, m[0], m[1], m[2], twp)
+  go((k_c2c_reg<AX, D, SG>), dim3((unsigned)((npen + reg_tp<SG>() - 1) / reg_tp<SG>())), dim3(DT), 0, A, m[0], m[1], m[2], twp, \
+     RealIO{})
 #define SH_C2CR_SEG(AX, D)                   \
   switch (seg) {                             \
     case 16: SH_C2CR(AX, D, 16); break;      \
@@ -879,6 +924,32 @@ struct surfhip_s_plan {
   }
   void fft3(int dir, float2 *A, Dims3 m) {
     for (int ax = 0; ax < 3; ++ax) launch_c2c(ax, dir, A, m, m[ax]);
+  }
+  // A = FFT_3(rin[i is] scale), the conversion folded into the first pass
+  // when every axis has a register-FFT length.
+  void fft3_from_real(float2 *A, Dims3 m, const float *rin, int64_t is, float scale) {
+    const int64_t n = (int64_t)m[0] * m[1] * m[2];
+    if (!(reg_len(m[0]) && reg_len(m[1]) && reg_len(m[2]))) {
+      go(k_to_complex, dim3(grid_for(n)), dim3(DT), 0, rin, is, A, n, scale);
+      fft3(-1, A, m);
+      return;
+    }
+    launch_c2c(0, -1, A, m, m[0], 1, RealIO{rin, nullptr, is, scale});
+    launch_c2c(1, -1, A, m, m[1]);
+    launch_c2c(2, -1, A, m, m[2]);
+  }
+  // rout[i os] = Re(IFFT_3(A))[i] scale (unnormalised inverse), the
+  // conversion folded into the last pass when it can be.
+  void ifft3_to_real(float2 *A, Dims3 m, float *rout, int64_t os, float scale) {
+    const int64_t n = (int64_t)m[0] * m[1] * m[2];
+    if (!(reg_len(m[0]) && reg_len(m[1]) && reg_len(m[2]))) {
+      fft3(1, A, m);
+      go(k_real_out, dim3(grid_for(n)), dim3(DT), 0, A, rout, os, scale, n);
+      return;
+    }
+    launch_c2c(0, 1, A, m, m[0]);
+    launch_c2c(1, 1, A, m, m[1]);
+    launch_c2c(2, 1, A, m, m[2], 2, RealIO{nullptr, rout, os, scale});
   }
   void zero(float2 *p, int64_t n) {
     go(k_dzero, dim3(grid_for(n)), dim3(DT), 0, p, n);
@@ -991,10 +1062,8 @@ struct surfhip_s_plan {
     use_lane(0);
     const int64_t os = nch;
     Dims3 n = T.dims;
-    const int64_t N = (int64_t)n[0] * n[1] * n[2];
     float2 *X = spectra[0].p;
-    go(k_to_complex, dim3(grid_for(N)), dim3(DT), 0, x, (int64_t)nch, X, N, 1.0f);
-    fft3(-1, X, n);
+    fft3_from_real(X, n, x, (int64_t)nch, 1.0f);
     for (int j = 0; j < T.J; ++j) {
       const PyrLevel &pl = T.pyr[j];
       float2 *low = spectra[j + 1].p;
@@ -1032,9 +1101,7 @@ struct surfhip_s_plan {
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    fft3(1, C, nl);
-    go(k_real_out, dim3(grid_for(NL)), dim3(DT), 0, C, coef + T.offsets[0] * nch, os,
-                       (float)(1.0 / (double)NL), NL);
+    ifft3_to_real(C, nl, coef + T.offsets[0] * nch, os, (float)(1.0 / (double)NL));
     join_k();
   }
 
@@ -1042,11 +1109,8 @@ struct surfhip_s_plan {
     use_lane(0);
     const int64_t is = nch;
     const Dims3 nl = T.pyr.back().nlow;
-    const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    go(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, coef + T.offsets[0] * nch, is, C, NL,
-                       1.0f);
-    fft3(-1, C, nl);
+    fft3_from_real(C, nl, coef + T.offsets[0] * nch, is, 1.0f);
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
@@ -1083,9 +1147,7 @@ struct surfhip_s_plan {
     }
     const Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
-    fft3(1, spectra[0].p, n);
-    go(k_real_out, dim3(grid_for(N)), dim3(DT), 0, spectra[0].p, x, (int64_t)nch,
-                       (float)(1.0 / (double)N), N);
+    ifft3_to_real(spectra[0].p, n, x, (int64_t)nch, (float)(1.0 / (double)N));
   }
 
   // x = A^T c, the adjoint of the analysis (not its inverse: the transform
@@ -1096,9 +1158,7 @@ struct surfhip_s_plan {
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
-    go(k_to_complex, dim3(grid_for(NL)), dim3(DT), 0, coef + T.offsets[0] * nch, is, C, NL,
-                       (float)(1.0 / (double)NL));
-    fft3(-1, C, nl);
+    fft3_from_real(C, nl, coef + T.offsets[0] * nch, is, (float)(1.0 / (double)NL));
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
@@ -1138,9 +1198,7 @@ struct surfhip_s_plan {
                          spectra[j + 1].p, pd, 1.0f / pd.D15);
     }
     const Dims3 n = T.dims;
-    const int64_t N = (int64_t)n[0] * n[1] * n[2];
-    fft3(1, spectra[0].p, n);
-    go(k_real_out, dim3(grid_for(N)), dim3(DT), 0, spectra[0].p, x, (int64_t)nch, 1.0f, N);
+    ifft3_to_real(spectra[0].p, n, x, (int64_t)nch, 1.0f);
   }
 };
 
