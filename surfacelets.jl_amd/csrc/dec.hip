@@ -196,11 +196,6 @@ __global__ __launch_bounds__(DT) void k_c2c_reg(float2 *A, int m1, int m2, int m
   }
 }
 
-__global__ void k_dzero(float2 *__restrict__ p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = make_float2(0.0f, 0.0f);
-}
-
 // X = complex(x) * scale, x read with element stride `is`.
 __global__ void k_to_complex(const float *__restrict__ x, int64_t is, float2 *__restrict__ X, int64_t n, float scale) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -234,20 +229,43 @@ __device__ __forceinline__ void pyr_factors(const PyrDev &P, int a, int b, int c
   s = sqrtf(r > 0.0f ? r : 0.0f);
 }
 
-// low[idx_low] = L X[idx] / D^1.5; X[idx] *= sqrt(1 - L^2).
+// Box coordinates along one axis whose lowpass index is l: the box is
+// 0..P then nl - P..nl - 1 of the lowpass axis (box coordinate a <= P, then
+// a > P), so l has none, one, or (when 2P = nl, at l = P) two, ascending.
+__device__ __forceinline__ int box_coords(int l, int nl, int P, int c[2]) {
+  int k = 0;
+  if (l <= P) c[k++] = l;
+  if (l >= nl - P) c[k++] = l - nl + 2 * P + 1;
+  return k;
+}
+
+// low[idx_low] = L X[idx] / D^1.5 and X[idx] *= sqrt(1 - L^2), as one pass
+// over the lowpass array, which also writes its zeros outside the box.  Each
+// box element of X belongs to exactly one lowpass element; where two box
+// elements share a lowpass element (2P = nl), the later one in box order
+// gives it its value, as the reference's array assignment does.
 __global__ void k_pyr_split(float2 *__restrict__ X, float2 *__restrict__ low, PyrDev P) {
-  const int64_t n = (int64_t)P.len1 * P.len2 * P.len3;
+  const int64_t n = (int64_t)P.l1 * P.l2 * P.l3;
+  const int P1 = (P.len1 - 1) / 2, P2 = (P.len2 - 1) / 2, P3 = (P.len3 - 1) / 2;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int a = e % P.len1;
-    const int64_t r = e / P.len1;
-    const int b = r % P.len2, c = r / P.len2;
-    int64_t xi, li;
-    float L, s;
-    pyr_factors(P, a, b, c, xi, li, L, s);
-    const float2 v = X[xi];
-    const float f = L / P.D15;
-    low[li] = make_float2(v.x * f, v.y * f);
-    X[xi] = make_float2(v.x * s, v.y * s);
+    const int i = (int)(e % P.l1);
+    const int64_t r = e / P.l1;
+    const int j = (int)(r % P.l2), k = (int)(r / P.l2);
+    int ca[2], cb[2], cc[2];
+    const int na = box_coords(i, P.l1, P1, ca), nb = box_coords(j, P.l2, P2, cb), nc = box_coords(k, P.l3, P3, cc);
+    float2 out = make_float2(0.0f, 0.0f);
+    for (int ia = 0; ia < na; ++ia)
+      for (int ib = 0; ib < nb; ++ib)
+        for (int ic = 0; ic < nc; ++ic) {
+          int64_t xi, li;
+          float L, s;
+          pyr_factors(P, ca[ia], cb[ib], cc[ic], xi, li, L, s);
+          const float2 v = X[xi];
+          const float f = L / P.D15;
+          out = make_float2(v.x * f, v.y * f);
+          X[xi] = make_float2(v.x * s, v.y * s);
+        }
+    low[e] = out;
   }
 }
 
@@ -951,9 +969,6 @@ struct surfhip_s_plan {
     launch_c2c(1, 1, A, m, m[1]);
     launch_c2c(2, 1, A, m, m[2], 2, RealIO{nullptr, rout, os, scale});
   }
-  void zero(float2 *p, int64_t n) {
-    go(k_dzero, dim3(grid_for(n)), dim3(DT), 0, p, n);
-  }
   static void transverse(int K, int &m1, int &m2) {
     m1 = K == 0 ? 1 : 0;
     m2 = K == 2 ? 1 : 2;
@@ -1067,9 +1082,8 @@ struct surfhip_s_plan {
     for (int j = 0; j < T.J; ++j) {
       const PyrLevel &pl = T.pyr[j];
       float2 *low = spectra[j + 1].p;
-      zero(low, (int64_t)pl.nlow[0] * pl.nlow[1] * pl.nlow[2]);
       const PyrDev &pd = lev[j]->pd;
-      go(k_pyr_split, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0,
+      go(k_pyr_split, dim3(grid_for((int64_t)pl.nlow[0] * pl.nlow[1] * pl.nlow[2])), dim3(DT), 0,
                          spectra[j].p, low, pd);
       const DecScaleT &s = T.scales[j];
       fork_k();
