@@ -126,8 +126,8 @@ int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs);
 int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x);
 
 /* Adjoints for reverse-mode rules (no counterpart functions in the
- * reference; the north-star rrule).  Each subband operator
- * irfft . M_k . rfft is self-adjoint (M_k real and even), so
+ * reference; a Julia caller would use them in a ChainRulesCore rrule).  Each
+ * subband operator irfft . M_k . rfft is self-adjoint (M_k real and even), so
  *   surfhip_ns_adjoint:            x = A^T c, the accumulation with the analysis
  *                                  masks (= synthesize! for a tight bank:
  *                                  src/transforms/ns_surfacelet.jl:21-25);
@@ -226,8 +226,9 @@ int surfhip_s_synthesize(surfhip_s_plan *plan, const void *coeffs, void *x);
 
 /* x = A^T c, the adjoint of analyze!.  The decimated transform is a frame but
  * not a tight one (src/transforms/surfacelet.jl:24-26), so this differs from
- * synthesize!; it is the backward of analyze in a reverse-mode rule (the
- * reference's north-star rrule).  No counterpart function in the reference. */
+ * synthesize!; it is the backward of analyze in a reverse-mode rule (a
+ * ChainRulesCore rrule on the Julia side).  No counterpart function in the
+ * reference. */
 int surfhip_s_adjoint(surfhip_s_plan *plan, const void *coeffs, void *x);
 
 #ifdef __cplusplus

@@ -5,8 +5,8 @@ The backward of each transform is its hand-written adjoint in libsurfhip:
   NSSurfacelet synthesize -> surfhip_ns_synthesize_adjoint (analyze if tight)
   Surfacelet analyze      -> surfhip_s_adjoint (the decimated transform is
                              not tight, so this is not synthesize)
-This is the rrule the reference's north star asks for, expressed for torch
-tensors.  The transforms are real-linear and act on complex data as two real
+These are ChainRulesCore-style rrules (the reverse-mode rule a Julia caller
+would register), expressed for torch tensors.  The transforms are real-linear and act on complex data as two real
 channels, so the same adjoint serves complex inputs.
 """
 import torch

@@ -1,4 +1,4 @@
-"""Adjoints and reverse-mode rules on the GPU (north-star item (a)).
+"""Adjoints and reverse-mode rules on the GPU.
 
 - NS: <A x, y> = <x, A^T y> and <S c, z> = <c, S^T z> for tight and not
   tight banks (dense random probes, FP32, relative 1e-5); for a tight bank

@@ -1,7 +1,7 @@
 """NSSurfacelet on the GPU through libsurfhip vs the FP64 oracle.
 
 Tolerances: FP32 results against the FP64 oracle, relative L2 over all
-subbands <= 1e-5 and per subband <= 1e-4 of that
+subbands <= 1e-5 (the FP32 accuracy bar) and per subband <= 1e-4 of that
 subband's norm or 1e-6 of the input norm, whichever is larger (subbands with
 almost no energy carry only rounding).  Structural properties the reference
 asserts bit for bit (streaming == materialised, complex == re + i im,
