@@ -196,6 +196,108 @@ __global__ __launch_bounds__(DT) void k_c2c_reg(float2 *A, int m1, int m2, int m
   }
 }
 
+// Both block FFTs of an hourglass K != 0 in one pass: a B1 x B2 tile of S
+// (axis 0, which is contiguous, and axis m2, for one index along K) is loaded
+// into LDS coalesced, transformed along axis 0 (one register FFT per tile
+// row) and then along m2 (one per tile column), and stored coalesced.
+// Block = (q1, q2, iK), q1 fastest.
+template <int B1, int B2> constexpr int blk2d_threads() {
+  constexpr int a = B2 * FourStep<B1>::R2, b = B1 * FourStep<B2>::R2;
+  return ((a > b ? a : b) + 63) / 64 * 64;
+}
+template <int B1, int B2> constexpr int blk2d_xch() {
+  constexpr int a = B2 * reg_ps<B1>(), b = B1 * reg_ps<B2>();
+  return a > b ? a : b;
+}
+
+template <int DIR, int B1, int B2>
+__global__ __launch_bounds__((blk2d_threads<B1, B2>())) void k_blk2d(float2 *S, int h0, int h1, int h2, int K, int m2,
+                                                                  const float2 *__restrict__ tw1g,
+                                                                  const float2 *__restrict__ tw2g) {
+  using FA = RegFFT<B1, DIR>;
+  using FB = RegFFT<B2, DIR>;
+  constexpr int NTH = blk2d_threads<B1, B2>(), TS = B1 + 1;
+  constexpr int RA1 = FA::R1, RA2 = FA::R2, RAR = FA::ROWS, PSA = reg_ps<B1>();
+  constexpr int RB1 = FB::R1, RB2 = FB::R2, RBR = FB::ROWS, PSB = reg_ps<B2>();
+  __shared__ float2 tile[B2 * TS];
+  __shared__ float2 xch[blk2d_xch<B1, B2>()];
+  __shared__ float2 twA[B1], twB[B2];
+  const int tid = threadIdx.x;
+  FA::fill_twiddles(twA, tw1g, tid, NTH);
+  FB::fill_twiddles(twB, tw2g, tid, NTH);
+  const int hd[3] = {h0, h1, h2};
+  const int n1 = h0 / B1, n2 = hd[m2] / B2;
+  const int q1 = (int)(blockIdx.x % n1);
+  const int rest = (int)(blockIdx.x / n1);
+  const int q2 = rest % n2, iK = rest / n2;
+  const int64_t s2 = m2 == 1 ? (int64_t)h0 : (int64_t)h0 * h1;
+  const int64_t sK = K == 1 ? (int64_t)h0 : (int64_t)h0 * h1;
+  const int64_t base = (int64_t)q1 * B1 + (int64_t)q2 * B2 * s2 + (int64_t)iK * sK;
+  for (int e = tid; e < B1 * B2; e += NTH) {
+    const int a = e % B1, b = e / B1;
+    tile[b * TS + a] = S[base + a + b * s2];
+  }
+  __syncthreads();  // tile and twiddles
+  {
+    // Along axis 0: pencil p = tile row p (B2 of them), RA2 consecutive lanes.
+    const int p = tid / RA2, j = tid % RA2;
+    const bool on = p < B2;
+    float2 v[RA1];
+    if (on) {
+#pragma unroll
+      for (int r = 0; r < RA1; ++r) v[r] = tile[p * TS + j + RA2 * r];
+      FA::phase1(v, j, twA);
+    }
+    float2 *pb = xch + (on ? p : 0) * PSA;
+    if (on) {
+#pragma unroll
+      for (int s = 0; s < RA1; ++s) pb[xpos_c2c<RA1>(s * RA2 + j)] = v[s];
+    }
+    __syncthreads();
+    if (on) {
+#pragma unroll
+      for (int e = 0; e < RA1; ++e) v[e] = pb[xpos_c2c<RA1>(j * RA1 + e)];
+      FA::phase2(v);
+#pragma unroll
+      for (int u = 0; u < RAR; ++u)
+#pragma unroll
+        for (int q = 0; q < RA2; ++q) tile[p * TS + (j * RAR + u) + RA1 * q] = v[u * RA2 + q];
+    }
+  }
+  __syncthreads();
+  {
+    // Along m2: pencil p = tile column p (B1 of them).
+    const int p = tid / RB2, j = tid % RB2;
+    const bool on = p < B1;
+    float2 v[RB1];
+    if (on) {
+#pragma unroll
+      for (int r = 0; r < RB1; ++r) v[r] = tile[(j + RB2 * r) * TS + p];
+      FB::phase1(v, j, twB);
+    }
+    float2 *pb = xch + (on ? p : 0) * PSB;
+    if (on) {
+#pragma unroll
+      for (int s = 0; s < RB1; ++s) pb[xpos_c2c<RB1>(s * RB2 + j)] = v[s];
+    }
+    __syncthreads();
+    if (on) {
+#pragma unroll
+      for (int e = 0; e < RB1; ++e) v[e] = pb[xpos_c2c<RB1>(j * RB1 + e)];
+      FB::phase2(v);
+#pragma unroll
+      for (int u = 0; u < RBR; ++u)
+#pragma unroll
+        for (int q = 0; q < RB2; ++q) tile[((j * RBR + u) + RB1 * q) * TS + p] = v[u * RB2 + q];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < B1 * B2; e += NTH) {
+    const int a = e % B1, b = e / B1;
+    S[base + a + b * s2] = tile[b * TS + a];
+  }
+}
+
 // X = complex(x) * scale, x read with element stride `is`.
 __global__ void k_to_complex(const float *__restrict__ x, int64_t is, float2 *__restrict__ X, int64_t n, float scale) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -943,6 +1045,46 @@ struct surfhip_s_plan {
   void fft3(int dir, float2 *A, Dims3 m) {
     for (int ax = 0; ax < 3; ++ax) launch_c2c(ax, dir, A, m, m[ax]);
   }
+  // The block FFTs of hourglass K on S (dims hd, block sizes bsize) along
+  // the two axes other than K: one tile pass when K != 0 and the tile fits,
+  // else one segmented pass per axis.
+  void block_ffts(int dir, float2 *S, Dims3 hd, Dims3 bsize, int K) {
+    const int m2 = K == 1 ? 2 : 1, B1 = bsize[0], B2 = bsize[m2];
+    if (K != 0 && reg_len(B1) && reg_len(B2) && B1 * B2 <= 4096) {
+      const int64_t nb = (int64_t)(hd[0] / B1) * (hd[m2] / B2) * hd[K];
+      const float2 *t1 = tw.p + plan_for(B1).tw, *t2 = tw.p + plan_for(B2).tw;
+#define SH_BLK(D, A, B) go((k_blk2d<D, A, B>), dim3((unsigned)nb), dim3(blk2d_threads<A, B>()), 0, S, hd[0], hd[1], hd[2], K, m2, t1, t2)
+#define SH_BLK_B(D, A)                  \
+  switch (B2) {                         \
+    case 16: SH_BLK(D, A, 16); break;   \
+    case 32: SH_BLK(D, A, 32); break;   \
+    case 64: SH_BLK(D, A, 64); break;   \
+    default: SH_BLK(D, A, 128); break;  \
+  }
+#define SH_BLK_A(D)                                 \
+  switch (B1) {                                     \
+    case 16: SH_BLK_B(D, 16) break;                 \
+    case 32: SH_BLK_B(D, 32) break;                 \
+    case 64:                                        \
+      switch (B2) {                                 \
+        case 16: SH_BLK(D, 64, 16); break;          \
+        case 32: SH_BLK(D, 64, 32); break;          \
+        default: SH_BLK(D, 64, 64); break;          \
+      }                                             \
+      break;                                        \
+    default:                                        \
+      if (B2 == 16) SH_BLK(D, 128, 16); else SH_BLK(D, 128, 32); \
+      break;                                        \
+  }
+      if (dir < 0) { SH_BLK_A(-1) } else { SH_BLK_A(1) }
+#undef SH_BLK_A
+#undef SH_BLK_B
+#undef SH_BLK
+      return;
+    }
+    for (int d = 0; d < 3; ++d)
+      if (d != K) launch_c2c(d, dir, S, hd, bsize[d]);
+  }
   // A = FFT_3(rin[i is] scale), the conversion folded into the first pass
   // when every axis has a register-FFT length.
   void fft3_from_real(float2 *A, Dims3 m, const float *rin, int64_t is, float scale) {
@@ -1106,8 +1248,7 @@ struct surfhip_s_plan {
         }
         Dims3 counts, bsize;
         scale_blocks(s, K, counts, bsize);
-        for (int d = 0; d < 3; ++d)
-          if (d != K) launch_c2c(d, 1, S, hd, bsize[d]);
+        block_ffts(1, S, hd, bsize, K);
         lines(true, j, K, S, coef, os);
       }
       use_lane(0);
@@ -1136,8 +1277,7 @@ struct surfhip_s_plan {
         lines(false, j, K, S, const_cast<float *>(coef), is);
         Dims3 counts, bsize;
         scale_blocks(s, K, counts, bsize);
-        for (int d = 0; d < 3; ++d)
-          if (d != K) launch_c2c(d, -1, S, hd, bsize[d]);
+        block_ffts(-1, S, hd, bsize, K);
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
         if (grouped(s, K)) {
           if (!s.stages[K].empty()) irc_group(s, K, 1, nullptr, nullptr, 1.0f, S);
@@ -1184,8 +1324,7 @@ struct surfhip_s_plan {
         lines(false, j, K, S, const_cast<float *>(coef), is, 1);
         Dims3 counts, bsize;
         scale_blocks(s, K, counts, bsize);
-        for (int d = 0; d < 3; ++d)
-          if (d != K) launch_c2c(d, -1, S, hd, bsize[d]);
+        block_ffts(-1, S, hd, bsize, K);
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
         if (grouped(s, K)) {
           if (!s.stages[K].empty()) irc_group(s, K, 2, nullptr, nullptr, 1.0f, S);
