@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE passes over this bench, each scaled by what the counter reads for
 # a plain 256 MiB device copy (profiles/ns128_hbm_bytes.md).  Measured once for
 # the current kernels; not measured live.
-TRAFFIC = {"plane_forward": 231638593, "plane_inverse": 204886243}
+TRAFFIC = {"plane_forward": 231427566, "plane_inverse": 281734204}
 DIMS = (128, 128, 128)
 SEED = 20260
 
