@@ -5,7 +5,7 @@
 // reads a[j + r n/R], applies the twiddles w^(r (j mod Ns) n/(Ns R)), runs an
 // R-point DFT in registers and writes b[(j/Ns) Ns R + (j mod Ns) + r Ns]
 // (decimation in time, self-sorting: no bit reversal), ping-ponging two LDS
-// buffers.  Radices 8, 4, 2, 3, 5 are hand-written; other primes up to
+// buffers.  Radices 8, 4, 2, 3, 5, 7 are hand-written; other primes up to
 // FFT_MAX_RADIX use a direct DFT.  Twiddles w^m = exp(-2 pi i m / n) are made
 // in FP64 on the host and rounded once to FP32.
 #pragma once
@@ -27,7 +27,7 @@ struct FftPlan1D {
   int32_t radix[FFT_MAX_PASSES];
 };
 
-// Host: factor n (radix 8 first, then 4, 2, 3, 5 and remaining primes) and
+// Host: factor n (radix 8 first, then 4, 2, 3, 5, 7 and remaining primes) and
 // append its twiddles to `pool`.  Throws for a prime factor > FFT_MAX_RADIX.
 FftPlan1D make_fft_plan(int n, std::vector<float2> &pool);
 
@@ -114,6 +114,29 @@ template <int DIR> __device__ __forceinline__ void dft5(float2 *v) {
   v[3] = csub(m2, n2);
 }
 
+template <int DIR> __device__ __forceinline__ void dft7(float2 *v) {
+  // cos/sin of 2 pi k / 7, k = 1, 2, 3.
+  const float c1 = 0.62348980185873353053f, c2 = -0.22252093395631440429f, c3 = -0.90096886790241912624f;
+  const float s1 = 0.78183148246802980871f, s2 = 0.97492791218182360702f, s3 = 0.43388373911755812048f;
+  const float2 a1 = cadd(v[1], v[6]), b1 = csub(v[1], v[6]);
+  const float2 a2 = cadd(v[2], v[5]), b2 = csub(v[2], v[5]);
+  const float2 a3 = cadd(v[3], v[4]), b3 = csub(v[3], v[4]);
+  const float2 m1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x + c3 * a3.x, v[0].y + c1 * a1.y + c2 * a2.y + c3 * a3.y);
+  const float2 m2 = make_float2(v[0].x + c2 * a1.x + c3 * a2.x + c1 * a3.x, v[0].y + c2 * a1.y + c3 * a2.y + c1 * a3.y);
+  const float2 m3 = make_float2(v[0].x + c3 * a1.x + c1 * a2.x + c2 * a3.x, v[0].y + c3 * a1.y + c1 * a2.y + c2 * a3.y);
+  // forward: X_m = M_m - i N_m, X_{7-m} = M_m + i N_m
+  const float2 n1 = rot90<DIR>(make_float2(s1 * b1.x + s2 * b2.x + s3 * b3.x, s1 * b1.y + s2 * b2.y + s3 * b3.y));
+  const float2 n2 = rot90<DIR>(make_float2(s2 * b1.x - s3 * b2.x - s1 * b3.x, s2 * b1.y - s3 * b2.y - s1 * b3.y));
+  const float2 n3 = rot90<DIR>(make_float2(s3 * b1.x - s1 * b2.x + s2 * b3.x, s3 * b1.y - s1 * b2.y + s2 * b3.y));
+  v[0] = cadd(v[0], cadd(a1, cadd(a2, a3)));
+  v[1] = cadd(m1, n1);
+  v[6] = csub(m1, n1);
+  v[2] = cadd(m2, n2);
+  v[5] = csub(m2, n2);
+  v[3] = cadd(m3, n3);
+  v[4] = csub(m3, n3);
+}
+
 // Direct R-point DFT for other primes, twiddles from the length-n table.
 template <int DIR> __device__ __noinline__ void dft_generic(float2 *v, int R, const float2 *tw, int n) {
   float2 out[FFT_MAX_RADIX];
@@ -150,6 +173,7 @@ template <int R, int DIR> __device__ __forceinline__ void fft_pass_fixed(
     else if (R == 3) dft3<DIR>(v);
     else if (R == 4) dft4<DIR>(v);
     else if (R == 5) dft5<DIR>(v);
+    else if (R == 7) dft7<DIR>(v);
     else if (R == 8) dft8<DIR>(v);
     const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
@@ -189,6 +213,7 @@ template <int DIR> __device__ float2 *lds_fft(float2 *a, float2 *b, const FftPla
       case 2: fft_pass_fixed<2, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
       case 3: fft_pass_fixed<3, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
       case 5: fft_pass_fixed<5, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
+      case 7: fft_pass_fixed<7, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
       default: fft_pass_generic<DIR>(a, b, n, R, G, Ns, tw, tid, nthreads); break;
     }
     __syncthreads();

@@ -15,7 +15,7 @@ FftPlan1D make_fft_plan(int n, std::vector<float2> &pool) {
   pl.n = n;
   pl.npass = 0;
   int m = n;
-  const int fixed[] = {8, 4, 2, 3, 5};
+  const int fixed[] = {8, 4, 2, 3, 5, 7};
   for (int r : fixed)
     while (m % r == 0) {
       if (pl.npass >= FFT_MAX_PASSES) fail(SURFHIP_ERR_ARG, fmt("FFT length %d has too many factors", n));
