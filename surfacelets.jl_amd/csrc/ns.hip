@@ -31,16 +31,26 @@ constexpr int NT = 256;
 
 // ------------------------------------------------------------------ kernels
 
-// Real rows of length n1 (row r at x[(r n1 + t) xs]) -> half-spectrum rows
-// out[r h1 + k].  Two rows share one complex FFT: z = a + i b, then
+// Row rp of a block's rows in plane-major order (rp = k2 n3 + k3) is the
+// real row (k2, k3), stored at row k3 n2 + k2 of x.
+__device__ __forceinline__ int64_t real_row(int rp, int n2, int n3) {
+  const int k2 = rp / n3;
+  return (int64_t)(rp - k2 * n3) * n2 + k2;
+}
+
+// Real rows of length n1 -> half-spectrum bins out[k R + rp], R = n2 n3.
+// A block takes G rows consecutive in rp, so the stores of each bin k are
+// runs of G words.  Two rows share one complex FFT: z = a + i b, then
 // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i.
 __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, int64_t xs, int64_t x_bs,
                                                  float2 *__restrict__ out, int64_t out_bs, int n1, int h1,
-                                                 int nrows, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+                                                 int n2, int n3, int G, FftPlan1D pl,
+                                                 const float2 *__restrict__ twg) {
   // Batch entry blockIdx.y: its own input and output.
   x += blockIdx.y * x_bs;
   out += blockIdx.y * out_bs;
   extern __shared__ float2 smem[];
+  const int nrows = n2 * n3;
   const int P = G / 2;
   float2 *tw = smem;
   float2 *a = tw + n1;
@@ -50,17 +60,17 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, in
   const int r0 = blockIdx.x * G;
   for (int idx = tid; idx < G * n1; idx += NT) {
     const int g = idx / n1, t = idx - g * n1;
-    const int r = r0 + g;
-    float v = r < nrows ? x[((int64_t)r * n1 + t) * xs] : 0.0f;
+    const int rp = r0 + g;
+    float v = rp < nrows ? x[(real_row(rp, n2, n3) * n1 + t) * xs] : 0.0f;
     float *ap = reinterpret_cast<float *>(&a[t * P + (g >> 1)]);
     ap[g & 1] = v;
   }
   __syncthreads();
   float2 *res = lds_fft<-1>(a, b, pl, P, tw, tid, NT);
   for (int idx = tid; idx < G * h1; idx += NT) {
-    const int g = idx / h1, k = idx - g * h1;
-    const int r = r0 + g;
-    if (r >= nrows) continue;
+    const int k = idx / G, g = idx - k * G;
+    const int rp = r0 + g;
+    if (rp >= nrows) continue;
     const int p = g >> 1;
     const float2 Z = res[k * P + p];
     const float2 Zc = conjf2(res[((n1 - k) % n1) * P + p]);
@@ -71,21 +81,22 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, in
       const float2 d = csub(Z, Zc);
       v = make_float2(0.5f * d.y, -0.5f * d.x);
     }
-    out[(int64_t)r * h1 + k] = v;
+    out[(int64_t)k * nrows + rp] = v;
   }
 }
 
-// Half-spectrum rows in[r h1 + k] -> real rows out[(r n1 + t) os] * scale.
-// Pairs of rows form z = a + i b through their Hermitian extensions; the
-// imaginary parts of the DC and (even n1) Nyquist bins are ignored, as a c2r
-// transform does.
+// Half-spectrum bins in[k R + rp] -> real rows (row (k2, k3) at
+// out[((k3 n2 + k2) n1 + t) os]) * scale.  Pairs of rows form z = a + i b
+// through their Hermitian extensions; the imaginary parts of the DC and
+// (even n1) Nyquist bins are ignored, as a c2r transform does.
 __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, int64_t in_bs,
                                                  float *__restrict__ out, int64_t os, int64_t out_bs, float scale,
-                                                 int n1, int h1, int nrows, int G, FftPlan1D pl,
+                                                 int n1, int h1, int n2, int n3, int G, FftPlan1D pl,
                                                  const float2 *__restrict__ twg) {
   in += blockIdx.y * in_bs;
   out += blockIdx.y * out_bs;
   extern __shared__ float2 smem[];
+  const int nrows = n2 * n3;
   const int P = G / 2;
   float2 *tw = smem;
   float2 *s = tw + n1;
@@ -95,9 +106,9 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
   for (int k = tid; k < n1; k += NT) tw[k] = twg[pl.tw + k];
   const int r0 = blockIdx.x * G;
   for (int idx = tid; idx < G * h1; idx += NT) {
-    const int g = idx / h1, k = idx - g * h1;
-    const int r = r0 + g;
-    s[idx] = r < nrows ? in[(int64_t)r * h1 + k] : make_float2(0.0f, 0.0f);
+    const int k = idx / G, g = idx - k * G;
+    const int rp = r0 + g;
+    s[g * h1 + k] = rp < nrows ? in[(int64_t)k * nrows + rp] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
   const bool even = (n1 & 1) == 0;
@@ -121,30 +132,29 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
   float2 *res = lds_fft<1>(a, b, pl, P, tw, tid, NT);
   for (int idx = tid; idx < G * n1; idx += NT) {
     const int g = idx / n1, t = idx - g * n1;
-    const int r = r0 + g;
-    if (r >= nrows) continue;
+    const int rp = r0 + g;
+    if (rp >= nrows) continue;
     const float2 v = res[t * P + (g >> 1)];
-    out[((int64_t)r * n1 + t) * os] = ((g & 1) ? v.y : v.x) * scale;
+    out[(real_row(rp, n2, n3) * n1 + t) * os] = ((g & 1) ? v.y : v.x) * scale;
   }
 }
 
-// Complex FFT along axis AXIS (2 or 3) of an rfft-grid array, G pencils per
-// block that are consecutive in k1.  MODE 0: out = FFT(in); MODE 1:
-// out = FFT(in . mask) (analysis); MODE 2: out += mask . FFT(in)
-// (synthesis).  Batch entry b = blockIdx.y reads in + b in_bs, writes
-// out + b out_bs and uses items[b].  MODE 3 (synthesis of a batch into one
-// sum): grid.y = 1 and the block runs entries 0..nb-1 in order, adding
-// mask_b . FFT(in_b) to out exactly as nb MODE 2 launches would.
-constexpr int AXIS_MAXE = 16;  // elements per thread: n G <= 4096 (the LDS limit)
+constexpr int AXIS_MAXE = 16;  // k_axis elements per thread: n G <= 4096
+
+// Complex FFT along axis AXIS (2 or 3) of a plane-major spectrum, G pencils
+// per block: for axis 3 G consecutive (k1, k2) rows (contiguous pencils),
+// for axis 2 G consecutive k3 of one k1 (pencils of stride n3).  MODE 0:
+// out = FFT(in); MODE 1: out = FFT(in . mask) (analysis); MODE 2:
+// out = mask . FFT(in) (a masked synthesis term).  Batch entry
+// b = blockIdx.y reads in + b in_bs, writes out + b out_bs and uses
+// items[b].  This is the path for planes that do not fit one block's LDS.
 template <int AXIS, int DIR, int MODE>
 __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, float2 *out, int64_t out_bs,
                                              int h1, int n2, int n3, int G, FftPlan1D pl,
                                              const float2 *__restrict__ twg, NSMaskView mv,
-                                             const NSItem *__restrict__ items, int nb) {
-  if (MODE != 3) {
-    in += blockIdx.y * in_bs;
-    out += blockIdx.y * out_bs;
-  }
+                                             const NSItem *__restrict__ items) {
+  in += blockIdx.y * in_bs;
+  out += blockIdx.y * out_bs;
   extern __shared__ float2 smem[];
   const int n = AXIS == 2 ? n2 : n3;
   float2 *tw = smem;
@@ -152,63 +162,228 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
   float2 *b = a + (size_t)n * G;
   const int tid = threadIdx.x;
   for (int k = tid; k < n; k += NT) tw[k] = twg[pl.tw + k];
-  const int ublocks = (h1 + G - 1) / G;
-  const int ub = blockIdx.x % ublocks;
-  const int v = blockIdx.x / ublocks;
-  const int u0 = ub * G;
-  const int64_t sa = AXIS == 2 ? (int64_t)h1 : (int64_t)h1 * n2;
-  const int64_t base = AXIS == 2 ? (int64_t)v * h1 * n2 : (int64_t)v * h1;
-  const int nent = MODE == 3 ? nb : 1;
-  float2 sum[AXIS_MAXE];
-  for (int e = 0; e < nent; ++e) {
-    const NSItem it = items[MODE == 3 ? e : blockIdx.y];
-    const float2 *src = in + (MODE == 3 ? e * in_bs : 0);
-    for (int idx = tid; idx < n * G; idx += NT) {
-      const int t = idx / G, g = idx - t * G;
-      const int k1 = u0 + g;
-      float2 val = make_float2(0.0f, 0.0f);
-      if (k1 < h1) {
-        val = src[base + k1 + t * sa];
-        if (MODE == 1) {
-          const int k2 = AXIS == 2 ? t : v, k3 = AXIS == 2 ? v : t;
-          val = cscale(val, ns_mask(mv, it, k1, k2, k3, true));
-        }
-      }
-      a[idx] = val;
-    }
-    __syncthreads();
-    float2 *res = lds_fft<DIR>(a, b, pl, G, tw, tid, NT);
-#pragma unroll
-    for (int m = 0; m < AXIS_MAXE; ++m) {
-      const int idx = tid + m * NT;
-      if (idx >= n * G) break;
-      const int t = idx / G, g = idx - t * G;
-      const int k1 = u0 + g;
-      if (k1 >= h1) continue;
-      const int64_t o = base + k1 + t * sa;
-      const float2 r = res[idx];
-      if (MODE == 2 || MODE == 3) {
-        const int k2 = AXIS == 2 ? t : v, k3 = AXIS == 2 ? v : t;
-        const float mk = ns_mask(mv, it, k1, k2, k3, false);
-        const float2 acc = MODE == 3 && e > 0 ? sum[m] : out[o];
-        const float2 nv = make_float2(acc.x + mk * r.x, acc.y + mk * r.y);
-        if (MODE == 2) out[o] = nv;
-        else sum[m] = nv;
-      } else {
-        out[o] = r;
-      }
-    }
-    __syncthreads();  // a and b are refilled by the next entry
+  const NSItem it = items[blockIdx.y];
+  // Pencil g of the block: first element at start + g gs, elements t es apart.
+  int k1, q0 = 0, u0 = 0, npen;
+  int64_t start, gs, es;
+  if (AXIS == 3) {
+    q0 = blockIdx.x * G;  // rows q = k1 n2 + k2
+    npen = h1 * n2 - q0;
+    k1 = 0;
+    start = (int64_t)q0 * n3;
+    gs = n3;
+    es = 1;
+  } else {
+    const int ublocks = (n3 + G - 1) / G;
+    k1 = blockIdx.x / ublocks;
+    u0 = (blockIdx.x - k1 * ublocks) * G;
+    npen = n3 - u0;
+    start = (int64_t)k1 * n2 * n3 + u0;
+    gs = 1;
+    es = n3;
   }
-  if (MODE == 3) {
-#pragma unroll
-    for (int m = 0; m < AXIS_MAXE; ++m) {
-      const int idx = tid + m * NT;
-      if (idx >= n * G) break;
-      const int t = idx / G, g = idx - t * G;
-      const int k1 = u0 + g;
-      if (k1 < h1) out[base + k1 + t * sa] = sum[m];
+  // Axis 3 maps consecutive threads along the pencil, axis 2 across pencils:
+  // both read and write runs of consecutive words.
+  auto coords = [&](int idx, int &t, int &g) {
+    if (AXIS == 3) {
+      g = idx / n;
+      t = idx - g * n;
+    } else {
+      t = idx / G;
+      g = idx - t * G;
     }
+  };
+  auto bin = [&](int t, int g, int &b1, int &b2, int &b3) {
+    if (AXIS == 3) {
+      const int q = q0 + g;
+      b1 = q / n2;
+      b2 = q - b1 * n2;
+      b3 = t;
+    } else {
+      b1 = k1;
+      b2 = t;
+      b3 = u0 + g;
+    }
+  };
+  for (int idx = tid; idx < n * G; idx += NT) {
+    int t, g;
+    coords(idx, t, g);
+    float2 val = make_float2(0.0f, 0.0f);
+    if (g < npen) {
+      val = in[start + g * gs + t * es];
+      if (MODE == 1) {
+        int b1, b2, b3;
+        bin(t, g, b1, b2, b3);
+        val = cscale(val, ns_mask(mv, it, b1, b2, b3, true));
+      }
+    }
+    a[t * G + g] = val;
+  }
+  __syncthreads();
+  float2 *res = lds_fft<DIR>(a, b, pl, G, tw, tid, NT);
+  for (int idx = tid; idx < n * G; idx += NT) {
+    int t, g;
+    coords(idx, t, g);
+    if (g >= npen) continue;
+    float2 r = res[t * G + g];
+    if (MODE == 2) {
+      int b1, b2, b3;
+      bin(t, g, b1, b2, b3);
+      r = cscale(r, ns_mask(mv, it, b1, b2, b3, false));
+    }
+    out[start + g * gs + t * es] = r;
+  }
+}
+
+// One radix-R Stockham pass over all npen pencils of one axis of an
+// LDS-resident plane, in place: pencil p's element t at P[p ps + t es].
+// Every thread reads the inputs of its (at most PLANE_V / R) butterflies
+// into registers, the block synchronises, and the outputs overwrite the
+// plane; the same index and twiddle rules as fft_pass_fixed.  The pass is a
+// call of its own (inlined together the six radices hold more registers
+// than a 1024-thread block has) and takes the plane and twiddles as offsets
+// into the block's dynamic LDS, so that its accesses stay LDS accesses.
+constexpr int PT = 1024;                 // threads of a plane block
+constexpr int PLANE_V = 16;              // values a thread holds across one pass
+constexpr size_t PLANE_LDS = 160 * 1024;  // LDS of one CU on gfx950
+// Division by d >= 1 as a multiply-high: q = floor(x / d) exactly for
+// 0 <= x < 2^16 and d <= 2^16 (m = floor((2^32 - 1) / d) + 1 exceeds 2^32 / d
+// by less than 1, so x m / 2^32 exceeds x / d by less than 2^-16 <= 1 / d).
+// d = 1 (m = 2^32) is taken apart.
+struct FastDiv {
+  uint32_t m;
+  int d;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) { return {0xFFFFFFFFu / (uint32_t)d + 1u, d}; }
+__device__ __forceinline__ int div_of(int x, FastDiv f) {
+  return f.d == 1 ? x : (int)__umulhi((uint32_t)x, f.m);
+}
+
+template <int R, int DIR>
+__device__ __noinline__ void plane_pass(int poff, int twoff, int n, int npen, int es, int ps, int Ns, int tid) {
+  extern __shared__ float2 smem[];
+  float2 *P = smem + poff;
+  const float2 *tw = smem + twoff;
+  constexpr int Q = PLANE_V / R;
+  const int nR = n / R;
+  const int tstep = n / (Ns * R);
+  const int nbf = npen * nR;
+  const FastDiv dpen = fast_div(npen), dns = fast_div(Ns);
+  float2 v[Q][R];
+  int at[Q];  // p ps + j es of each butterfly
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int w = tid + q * PT;
+    if (w < nbf) {
+      const int j = div_of(w, dpen), p = w - j * npen;
+      at[q] = p * ps + j * es;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = P[at[q] + r * nR * es];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int w = tid + q * PT;
+    if (w < nbf) {
+      const int j = div_of(w, dpen);
+      const int jn = div_of(j, dns);
+      const int k = j - jn * Ns;
+      if (k) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twid<DIR>(tw, r * k * tstep));
+      }
+      if (R == 2) dft2<DIR>(v[q]);
+      else if (R == 3) dft3<DIR>(v[q]);
+      else if (R == 4) dft4<DIR>(v[q]);
+      else if (R == 5) dft5<DIR>(v[q]);
+      else if (R == 7) dft7<DIR>(v[q]);
+      else if (R == 8) dft8<DIR>(v[q]);
+      // output index jn Ns R + k = j + jn Ns (R - 1)
+      const int o = at[q] + jn * Ns * (R - 1) * es;
+#pragma unroll
+      for (int r = 0; r < R; ++r) P[o + r * Ns * es] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Whether radix R fits the plane pass for npen pencils of length n.
+inline bool plane_radix_fits(int R, int64_t elems) {
+  const bool known = R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8;
+  return known && (elems / R + PT - 1) / PT <= PLANE_V / R;
+}
+
+template <int DIR>
+__device__ void plane_fft(int poff, int twoff, const FftPlan1D &pl, int npen, int es, int ps, int tid) {
+  int Ns = 1;
+  for (int p = 0; p < pl.npass; ++p) {
+    const int R = pl.radix[p];
+    switch (R) {
+      case 8: plane_pass<8, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 4: plane_pass<4, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 2: plane_pass<2, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 3: plane_pass<3, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 5: plane_pass<5, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      default: plane_pass<7, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+    }
+    Ns *= R;
+  }
+}
+
+// 2-D FFT over the (k2, k3) plane of one k1 (blockIdx.x) of a plane-major
+// spectrum, the plane held in LDS with row pitch n3p (odd, so that strided
+// pencils fall in distinct banks).  Modes as k_axis: 0 plain, 1 mask before
+// the (inverse) transform, 2 mask after the (forward) transform; batch
+// entry blockIdx.y.  Replaces one k_axis pass per axis.  TWO: built for two
+// blocks per CU (64 VGPRs), for planes whose LDS lets two blocks share a CU.
+template <int DIR, int MODE, bool TWO>
+__global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_plane(const float2 *in, int64_t in_bs, float2 *out, int64_t out_bs,
+                                              int n2, int n3, int n3p, FftPlan1D p2, FftPlan1D p3,
+                                              const float2 *__restrict__ twg, NSMaskView mv,
+                                              const NSItem *__restrict__ items) {
+  in += blockIdx.y * in_bs;
+  out += blockIdx.y * out_bs;
+  extern __shared__ float2 smem[];
+  float2 *t2 = smem;
+  float2 *t3 = t2 + n2;
+  float2 *P = t3 + n3;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < n2; k += PT) t2[k] = twg[p2.tw + k];
+  for (int k = tid; k < n3; k += PT) t3[k] = twg[p3.tw + k];
+  const int k1 = blockIdx.x;
+  const int nel = n2 * n3;
+  const int64_t base = (int64_t)k1 * nel;
+  const NSItem it = items[blockIdx.y];
+  const FastDiv d3 = fast_div(n3);
+  for (int i = tid; i < nel; i += PT) {
+    const int k2 = div_of(i, d3), k3 = i - k2 * n3;
+    float2 v = in[base + i];
+    if (MODE == 1) v = cscale(v, ns_mask(mv, it, k1, k2, k3, true));
+    P[k2 * n3p + k3] = v;
+  }
+  __syncthreads();
+  plane_fft<DIR>(n2 + n3, n2, p3, n2, 1, n3p, tid);  // along k3: pencils are rows
+  plane_fft<DIR>(n2 + n3, 0, p2, n3, n3p, 1, tid);   // along k2: pencils are columns
+  for (int i = tid; i < nel; i += PT) {
+    const int k2 = div_of(i, d3), k3 = i - k2 * n3;
+    float2 v = P[k2 * n3p + k3];
+    if (MODE == 2) v = cscale(v, ns_mask(mv, it, k1, k2, k3, false));
+    out[base + i] = v;
+  }
+}
+
+// acc += W_0 + W_1 + ... + W_{nb-1}, added in that order (W_b at W + b bs),
+// so a batch sums exactly as nb single-entry calls do.
+__global__ __launch_bounds__(NT) void k_sum(float2 *__restrict__ acc, const float2 *__restrict__ W, int64_t bs,
+                                            int nb, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+    float2 s = acc[i];
+    for (int b = 0; b < nb; ++b) {
+      const float2 w = W[b * bs + i];
+      s = make_float2(s.x + w.x, s.y + w.y);
+    }
+    acc[i] = s;
   }
 }
 
@@ -252,6 +427,8 @@ struct surfhip_ns_plan {
   int n1 = 0, n2 = 0, n3 = 0, h1 = 0;
   FftPlan1D f1{}, f2{}, f3{};
   int g_rows = 16, g_axis = 16;
+  int n3p = 1;        // plane row pitch: n3 rounded up to odd
+  bool plane = false;  // axes 2 and 3 by the plane kernel
   int nwork = 1;                    // subbands per general-path launch (work holds nwork spectra)
   float tuned_ms = 0.0f;            // timed sequence of the chosen configuration (0: not tuned)
   DevBuf<float> gana, gsyn, planes;
@@ -524,31 +701,69 @@ struct surfhip_ns_plan {
   size_t lds_axis(int n) const { return sizeof(float2) * ((size_t)n + 2 * (size_t)n * g_axis); }
 
   int rows() const { return n2 * n3; }
+  // The plane kernel's LDS: both axes' twiddles and one (k2, k3) plane of
+  // row pitch n3p.
+  size_t lds_plane() const { return sizeof(float2) * ((size_t)n2 + n3 + (size_t)n2 * n3p); }
+  bool plane_two() const { return 2 * lds_plane() <= PLANE_LDS; }
+  // Whether the plane kernel takes axes 2 and 3: the plane fits one block's
+  // LDS and every radix of both plans has a hand-written pass that holds its
+  // butterflies in registers.
+  bool plane_fits() const {
+    if (lds_plane() > PLANE_LDS) return false;
+    for (const FftPlan1D *pl : {&f2, &f3})
+      for (int k = 0; k < pl->npass; ++k)
+        if (!plane_radix_fits(pl->radix[k], (int64_t)n2 * n3)) return false;
+    return true;
+  }
 
   // General-path launchers; nb > 1 runs a batch of nb subbands (grid.y),
   // entry b at in + b in_bs / out + b out_bs with item item0 + b.
   void r2c(const float *x, int64_t xs, float2 *out, int nb = 1, int64_t x_bs = 0, int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
     hipLaunchKernelGGL(k_rows_r2c, dim3(grid, nb), dim3(NT), lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1,
-                       h1, rows(), g_rows, f1, tw.p);
+                       h1, n2, n3, g_rows, f1, tw.p);
     SH_LAUNCHED();
   }
   void c2r(const float2 *in, float *out, int64_t os, float scale, int nb = 1, int64_t in_bs = 0,
            int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
     hipLaunchKernelGGL(k_rows_c2r, dim3(grid, nb), dim3(NT), lds_rows_c2r(), stream, in, in_bs, out, os, out_bs,
-                       scale, n1, h1, rows(), g_rows, f1, tw.p);
+                       scale, n1, h1, n2, n3, g_rows, f1, tw.p);
     SH_LAUNCHED();
   }
   template <int AXIS, int DIR, int MODE>
   void axis(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
-    const int ublocks = (h1 + g_axis - 1) / g_axis;
-    const int grid = ublocks * (AXIS == 2 ? n3 : n2);
+    const int G = g_axis;
+    const int grid = AXIS == 3 ? (h1 * n2 + G - 1) / G : h1 * ((n3 + G - 1) / G);
     const FftPlan1D &pl = AXIS == 2 ? f2 : f3;
-    if ((int64_t)pl.n * g_axis > (int64_t)AXIS_MAXE * NT)
-      fail(SURFHIP_ERR_ARG, fmt("axis block of %d x %d elements exceeds %d", pl.n, g_axis, AXIS_MAXE * NT));
-    hipLaunchKernelGGL((k_axis<AXIS, DIR, MODE>), dim3(grid, MODE == 3 ? 1 : nb), dim3(NT), lds_axis(pl.n), stream,
-                       in, in_bs, out, out_bs, h1, n2, n3, g_axis, pl, tw.p, dview, ditems.p + item0, nb);
+    if ((int64_t)pl.n * G > (int64_t)AXIS_MAXE * NT)
+      fail(SURFHIP_ERR_ARG, fmt("axis block of %d x %d elements exceeds %d", pl.n, G, AXIS_MAXE * NT));
+    hipLaunchKernelGGL((k_axis<AXIS, DIR, MODE>), dim3(grid, nb), dim3(NT), lds_axis(pl.n), stream, in, in_bs, out,
+                       out_bs, h1, n2, n3, G, pl, tw.p, dview, ditems.p + item0);
+    SH_LAUNCHED();
+  }
+  // FFT over axes 2 and 3 (DIR), modes as k_axis: the plane kernel where the
+  // plane fits, else one k_axis pass per axis (the mask on the pass that
+  // touches the spectrum first (MODE 1) or last (MODE 2)).
+  template <int DIR, int MODE>
+  void axes23(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
+    if (plane) {
+      if (plane_two())
+        hipLaunchKernelGGL((k_plane<DIR, MODE, true>), dim3(h1, nb), dim3(PT), lds_plane(), stream, in, in_bs, out,
+                           out_bs, n2, n3, n3p, f2, f3, tw.p, dview, ditems.p + item0);
+      else
+        hipLaunchKernelGGL((k_plane<DIR, MODE, false>), dim3(h1, nb), dim3(PT), lds_plane(), stream, in, in_bs, out,
+                           out_bs, n2, n3, n3p, f2, f3, tw.p, dview, ditems.p + item0);
+      SH_LAUNCHED();
+      return;
+    }
+    axis<3, DIR, MODE == 1 ? 1 : 0>(in, out, item0, nb, in_bs, out_bs);
+    axis<2, DIR, MODE == 2 ? 2 : 0>(out, out, item0, nb, out_bs, out_bs);
+  }
+  // acc += work_0 + ... + work_{nb-1} in order.
+  void sum_into(float2 *a, const float2 *w, int nb) {
+    const int64_t blocks = std::min<int64_t>((H + NT - 1) / NT, 4096);
+    hipLaunchKernelGGL(k_sum, dim3((unsigned)blocks), dim3(NT), 0, stream, a, w, H, nb, H);
     SH_LAUNCHED();
   }
   void zero(float2 *p, int64_t n) {
@@ -607,8 +822,9 @@ struct surfhip_ns_plan {
       }
       *knob = pick;
     };
-    try_knob(&g_axis, {1, 2, 4, 8, 16, 32},
-             [&] { return lds_axis(n2) <= 65536 && lds_axis(n3) <= 65536; });
+    if (!plane)
+      try_knob(&g_axis, {1, 2, 4, 8, 16, 32},
+               [&] { return lds_axis(n2) <= 65536 && lds_axis(n3) <= 65536; });
     try_knob(&g_rows, {2, 4, 8, 16, 32},
              [&] { return lds_rows_r2c() <= 65536 && lds_rows_c2r() <= 65536; });
     for (FftPlan1D *pl : {&f1, &f2, &f3}) {
@@ -634,8 +850,7 @@ struct surfhip_ns_plan {
         continue;
       }
       r2c(x + ch, nch, X[ch].p);
-      axis<2, -1, 0>(X[ch].p, X[ch].p);
-      axis<3, -1, 0>(X[ch].p, X[ch].p);
+      axes23<-1, 0>(X[ch].p, X[ch].p);
     }
   }
   // Generic path: subbands item0 .. item0+nw-1 into out (subband stride
@@ -645,23 +860,20 @@ struct surfhip_ns_plan {
     for (int ch = 0; ch < nch; ++ch)
       for (int w0 = 0; w0 < nw; w0 += nwork) {
         const int nb = std::min(nwork, nw - w0);
-        axis<3, 1, 1>(X[ch].p, work.p, item0 + w0, nb, 0, H);
-        axis<2, 1, 0>(work.p, work.p, 0, nb, H, H);
+        axes23<1, 1>(X[ch].p, work.p, item0 + w0, nb, 0, H);
         c2r(work.p, out + (int64_t)w0 * N * nch + ch, nch, scale, nb, H, N * nch);
       }
   }
   // Generic path: acc += sum over subbands item0 .. item0+nw-1 (in order) of
-  // rfft(band) . M_i, bands at in (subband stride N nch).
+  // rfft(band) . M_i, bands at in (subband stride N nch): the masked spectra
+  // of a batch go to work, then one pass adds them to acc in order.
   void accumulate_generic(int item0, int nw, const float *in) {
     for (int ch = 0; ch < nch; ++ch)
       for (int w0 = 0; w0 < nw; w0 += nwork) {
         const int nb = std::min(nwork, nw - w0);
         r2c(in + (int64_t)w0 * N * nch + ch, nch, work.p, nb, N * nch, H);
-        axis<2, -1, 0>(work.p, work.p, 0, nb, H, H);
-        if (nb == 1)
-          axis<3, -1, 2>(work.p, acc[ch].p, item0 + w0);
-        else
-          axis<3, -1, 3>(work.p, acc[ch].p, item0 + w0, nb, H, 0);
+        axes23<-1, 2>(work.p, work.p, item0 + w0, nb, H, H);
+        sum_into(acc[ch].p, work.p, nb);
       }
   }
   void make_aux() {
@@ -818,8 +1030,7 @@ struct surfhip_ns_plan {
       return;
     }
     for (int ch = 0; ch < nch; ++ch) {
-      axis<3, 1, 0>(acc[ch].p, work.p);
-      axis<2, 1, 0>(work.p, work.p);
+      axes23<1, 0>(acc[ch].p, work.p);
       c2r(work.p, x + ch, nch, 1.0f / (float)N);
     }
   }
@@ -898,6 +1109,19 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       check_lds(P->lds_rows_c2r(), "the axis-1 transform");
       check_lds(P->lds_axis(n2), "the axis-2 transform");
       check_lds(P->lds_axis(n3), "the axis-3 transform");
+      // Axes 2 and 3 by the plane kernel where it fits (SURFHIP_NS_NO_PLANE
+      // selects the per-axis passes, for comparison).
+      P->n3p = n3 | 1;
+      P->plane = std::getenv("SURFHIP_NS_NO_PLANE") == nullptr && P->plane_fits();
+      if (P->plane) {
+        const int lds = (int)P->lds_plane();
+        for (const void *k :
+             {(const void *)&k_plane<-1, 0, false>, (const void *)&k_plane<1, 0, false>,
+              (const void *)&k_plane<1, 1, false>, (const void *)&k_plane<-1, 2, false>,
+              (const void *)&k_plane<-1, 0, true>, (const void *)&k_plane<1, 0, true>,
+              (const void *)&k_plane<1, 1, true>, (const void *)&k_plane<-1, 2, true>})
+          SH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      }
     }
     P->tw.upload(pool);
     {
