@@ -113,10 +113,9 @@ int surfhip_ns_set_stream(surfhip_ns_plan *plan, void *stream);
  * flags play that role on the CPU).  Here it runs the plan's autotuner: for
  * sizes outside the factored kernels' grids it times the plan's own launch
  * sequence on its stream and keeps the fastest block shapes and radix orders
- * (synchronises the stream).  config (NULL or 7 int32): factored (1/0),
- * rows per axis-1 block, pencils per axis-2/3 block, each axis's radix
- * order as decimal digit pairs (e.g. 804 = radix 8 then 4), and pencils per
- * block of the batched axis-3 accumulation of a synthesis. */
+ * (synchronises the stream).  config (NULL or 6 int32): factored (1/0),
+ * rows per axis-1 block, pencils per axis-2/3 block, and each axis's radix
+ * order as decimal digit pairs (e.g. 804 = radix 8 then 4). */
 int surfhip_ns_plan_tune(surfhip_ns_plan *plan, int32_t *config);
 
 /* analyze!: src/transforms/ns_surfacelet.jl:551-566.  x: n1 n2 n3 volume,

@@ -169,7 +169,7 @@ class NSSurfaceletWorkspace:
         """Run the plan's autotuner (block shapes and radix orders of the
         general-size kernels; the factored kernels have nothing to tune) and
         return the configuration it keeps."""
-        cfg = (ctypes.c_int32 * 7)()
+        cfg = (ctypes.c_int32 * 6)()
         self.bind_stream()
         _lib.call("surfhip_ns_plan_tune", self._plan, cfg)
         def radices(code):
@@ -179,8 +179,7 @@ class NSSurfaceletWorkspace:
                 code //= 100
             return out[::-1]
         return {"factored": bool(cfg[0]), "rows_per_block": int(cfg[1]), "pencils_per_block": int(cfg[2]),
-                "radix_orders": [radices(int(cfg[3 + a])) for a in range(3)],
-                "pencils_per_accumulation_block": int(cfg[6])}
+                "radix_orders": [radices(int(cfg[3 + a])) for a in range(3)]}
 
     def set_timing(self, enable):
         """Bracket every kernel launch with HIP events (a benchmarking hook)."""
