@@ -31,6 +31,19 @@ struct FftPlan1D {
 // append its twiddles to `pool`.  Throws for a prime factor > FFT_MAX_RADIX.
 FftPlan1D make_fft_plan(int n, std::vector<float2> &pool);
 
+// Division by d >= 1 as a multiply-high: q = floor(x / d) exactly for
+// 0 <= x < 2^16 and d <= 2^16 (m = floor((2^32 - 1) / d) + 1 exceeds 2^32 / d
+// by less than 1, so x m / 2^32 exceeds x / d by less than 2^-16 <= 1 / d).
+// d = 1 (m = 2^32) is taken apart.
+struct FastDiv {
+  uint32_t m;
+  int d;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) { return {0xFFFFFFFFu / (uint32_t)d + 1u, d}; }
+__device__ __forceinline__ int div_of(int x, FastDiv f) {
+  return f.d == 1 ? x : (int)__umulhi((uint32_t)x, f.m);
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -155,16 +168,17 @@ template <int DIR> __device__ __noinline__ void dft_generic(float2 *v, int R, co
 }
 
 template <int R, int DIR> __device__ __forceinline__ void fft_pass_fixed(
-    const float2 *__restrict__ a, float2 *__restrict__ b, int n, int G, int Ns, const float2 *tw,
-    int tid, int nthreads) {
+    const float2 *__restrict__ a, float2 *__restrict__ b, int n, int G, FastDiv dG, int Ns, FastDiv dNs,
+    const float2 *tw, int tid, int nthreads) {
   const int nR = n / R;
   const int tstep = n / (Ns * R);
   for (int w = tid; w < G * nR; w += nthreads) {
-    const int g = w % G, j = w / G;
+    const int j = div_of(w, dG), g = w - j * G;
     float2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = a[(j + r * nR) * G + g];
-    const int k = j % Ns;
+    const int jn = div_of(j, dNs);
+    const int k = j - jn * Ns;
     if (k) {
 #pragma unroll
       for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<DIR>(tw, r * k * tstep));
@@ -175,7 +189,7 @@ template <int R, int DIR> __device__ __forceinline__ void fft_pass_fixed(
     else if (R == 5) dft5<DIR>(v);
     else if (R == 7) dft7<DIR>(v);
     else if (R == 8) dft8<DIR>(v);
-    const int idxD = (j / Ns) * Ns * R + k;
+    const int idxD = jn * Ns * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) b[(idxD + r * Ns) * G + g] = v[r];
   }
@@ -198,23 +212,41 @@ template <int DIR> __device__ __noinline__ void fft_pass_generic(
   }
 }
 
-// Transform the G pencils in `a` (LDS), using `b` as the other half of the
-// ping-pong pair; returns the buffer holding the result.  `tw` holds the
-// plan's n twiddles (LDS or global).  All threads of the block must call it.
-template <int DIR> __device__ float2 *lds_fft(float2 *a, float2 *b, const FftPlan1D &pl, int G,
-                                              const float2 *tw, int tid, int nthreads) {
-  int Ns = 1;
-  const int n = pl.n;
+// Whether a plan has a radix without a hand-written butterfly (it then
+// needs lds_fft's direct-DFT pass).
+inline bool fft_plan_generic(const FftPlan1D &pl) {
   for (int p = 0; p < pl.npass; ++p) {
     const int R = pl.radix[p];
+    if (R != 2 && R != 3 && R != 4 && R != 5 && R != 7 && R != 8) return true;
+  }
+  return false;
+}
+
+// Transform the G pencils in `a` (LDS), using `b` as the other half of the
+// ping-pong pair; returns the buffer holding the result.  `tw` holds the
+// plan's n twiddles (LDS or global).  All threads of the block must call it;
+// n G < 2^16.  GEN = false leaves out the direct-DFT pass (a call, whose
+// register reservation would cap the kernel's occupancy): only for plans
+// with fft_plan_generic(pl) false.
+template <int DIR, bool GEN = true>
+__device__ float2 *lds_fft(float2 *a, float2 *b, const FftPlan1D &pl, int G, const float2 *tw, int tid,
+                           int nthreads) {
+  int Ns = 1;
+  const int n = pl.n;
+  const FastDiv dG = fast_div(G);
+  for (int p = 0; p < pl.npass; ++p) {
+    const int R = pl.radix[p];
+    const FastDiv dNs = fast_div(Ns);
     switch (R) {
-      case 8: fft_pass_fixed<8, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
-      case 4: fft_pass_fixed<4, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
-      case 2: fft_pass_fixed<2, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
-      case 3: fft_pass_fixed<3, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
-      case 5: fft_pass_fixed<5, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
-      case 7: fft_pass_fixed<7, DIR>(a, b, n, G, Ns, tw, tid, nthreads); break;
-      default: fft_pass_generic<DIR>(a, b, n, R, G, Ns, tw, tid, nthreads); break;
+      case 8: fft_pass_fixed<8, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 4: fft_pass_fixed<4, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 2: fft_pass_fixed<2, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 3: fft_pass_fixed<3, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 5: fft_pass_fixed<5, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 7: fft_pass_fixed<7, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      default:
+        if (GEN) fft_pass_generic<DIR>(a, b, n, R, G, Ns, tw, tid, nthreads);
+        break;
     }
     __syncthreads();
     float2 *t = a;

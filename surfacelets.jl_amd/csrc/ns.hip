@@ -32,16 +32,21 @@ constexpr int NT = 256;
 // ------------------------------------------------------------------ kernels
 
 // Row rp of a block's rows in plane-major order (rp = k2 n3 + k3) is the
-// real row (k2, k3), stored at row k3 n2 + k2 of x.
-__device__ __forceinline__ int64_t real_row(int rp, int n2, int n3) {
-  const int k2 = rp / n3;
-  return (int64_t)(rp - k2 * n3) * n2 + k2;
+// real row (k2, k3), stored at row k3 n2 + k2 of x.  The G row offsets of a
+// block are made once, into LDS.
+__device__ __forceinline__ void real_rows(int64_t *roff, int r0, int G, int n2, int n3, int64_t n1, int tid) {
+  for (int g = tid; g < G; g += NT) {
+    const int rp = r0 + g;
+    const int k2 = rp / n3;
+    roff[g] = ((int64_t)(rp - k2 * n3) * n2 + k2) * n1;
+  }
 }
 
 // Real rows of length n1 -> half-spectrum bins out[k R + rp], R = n2 n3.
 // A block takes G rows consecutive in rp, so the stores of each bin k are
 // runs of G words.  Two rows share one complex FFT: z = a + i b, then
 // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i.
+template <bool GEN>
 __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, int64_t xs, int64_t x_bs,
                                                  float2 *__restrict__ out, int64_t out_bs, int n1, int h1,
                                                  int n2, int n3, int G, FftPlan1D pl,
@@ -55,20 +60,24 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, in
   float2 *tw = smem;
   float2 *a = tw + n1;
   float2 *b = a + (size_t)n1 * P;
+  int64_t *roff = reinterpret_cast<int64_t *>(b + (size_t)n1 * P);
   const int tid = threadIdx.x;
   for (int k = tid; k < n1; k += NT) tw[k] = twg[pl.tw + k];
   const int r0 = blockIdx.x * G;
+  real_rows(roff, r0, G, n2, n3, n1, tid);
+  __syncthreads();
+  const FastDiv dn1 = fast_div(n1), dG = fast_div(G);
   for (int idx = tid; idx < G * n1; idx += NT) {
-    const int g = idx / n1, t = idx - g * n1;
+    const int g = div_of(idx, dn1), t = idx - g * n1;
     const int rp = r0 + g;
-    float v = rp < nrows ? x[(real_row(rp, n2, n3) * n1 + t) * xs] : 0.0f;
+    float v = rp < nrows ? x[(roff[g] + t) * xs] : 0.0f;
     float *ap = reinterpret_cast<float *>(&a[t * P + (g >> 1)]);
     ap[g & 1] = v;
   }
   __syncthreads();
-  float2 *res = lds_fft<-1>(a, b, pl, P, tw, tid, NT);
+  float2 *res = lds_fft<-1, GEN>(a, b, pl, P, tw, tid, NT);
   for (int idx = tid; idx < G * h1; idx += NT) {
-    const int k = idx / G, g = idx - k * G;
+    const int k = div_of(idx, dG), g = idx - k * G;
     const int rp = r0 + g;
     if (rp >= nrows) continue;
     const int p = g >> 1;
@@ -89,6 +98,7 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, in
 // out[((k3 n2 + k2) n1 + t) os]) * scale.  Pairs of rows form z = a + i b
 // through their Hermitian extensions; the imaginary parts of the DC and
 // (even n1) Nyquist bins are ignored, as a c2r transform does.
+template <bool GEN>
 __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, int64_t in_bs,
                                                  float *__restrict__ out, int64_t os, int64_t out_bs, float scale,
                                                  int n1, int h1, int n2, int n3, int G, FftPlan1D pl,
@@ -102,18 +112,21 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
   float2 *s = tw + n1;
   float2 *a = s + (size_t)G * h1;
   float2 *b = a + (size_t)n1 * P;
+  int64_t *roff = reinterpret_cast<int64_t *>(b + (size_t)n1 * P);
   const int tid = threadIdx.x;
   for (int k = tid; k < n1; k += NT) tw[k] = twg[pl.tw + k];
   const int r0 = blockIdx.x * G;
+  real_rows(roff, r0, G, n2, n3, n1, tid);
+  const FastDiv dn1 = fast_div(n1), dG = fast_div(G), dP = fast_div(P);
   for (int idx = tid; idx < G * h1; idx += NT) {
-    const int k = idx / G, g = idx - k * G;
+    const int k = div_of(idx, dG), g = idx - k * G;
     const int rp = r0 + g;
     s[g * h1 + k] = rp < nrows ? in[(int64_t)k * nrows + rp] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
   const bool even = (n1 & 1) == 0;
   for (int idx = tid; idx < P * n1; idx += NT) {
-    const int t = idx / P, p = idx - t * P;
+    const int t = div_of(idx, dP), p = idx - t * P;
     float2 A, B;
     if (t < h1) {
       A = s[(2 * p) * h1 + t];
@@ -129,13 +142,13 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
     a[t * P + p] = make_float2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  float2 *res = lds_fft<1>(a, b, pl, P, tw, tid, NT);
+  float2 *res = lds_fft<1, GEN>(a, b, pl, P, tw, tid, NT);
   for (int idx = tid; idx < G * n1; idx += NT) {
-    const int g = idx / n1, t = idx - g * n1;
+    const int g = div_of(idx, dn1), t = idx - g * n1;
     const int rp = r0 + g;
     if (rp >= nrows) continue;
     const float2 v = res[t * P + (g >> 1)];
-    out[(real_row(rp, n2, n3) * n1 + t) * os] = ((g & 1) ? v.y : v.x) * scale;
+    out[(roff[g] + t) * os] = ((g & 1) ? v.y : v.x) * scale;
   }
 }
 
@@ -148,7 +161,7 @@ constexpr int AXIS_MAXE = 16;  // k_axis elements per thread: n G <= 4096
 // out = mask . FFT(in) (a masked synthesis term).  Batch entry
 // b = blockIdx.y reads in + b in_bs, writes out + b out_bs and uses
 // items[b].  This is the path for planes that do not fit one block's LDS.
-template <int AXIS, int DIR, int MODE>
+template <int AXIS, int DIR, int MODE, bool GEN>
 __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, float2 *out, int64_t out_bs,
                                              int h1, int n2, int n3, int G, FftPlan1D pl,
                                              const float2 *__restrict__ twg, NSMaskView mv,
@@ -220,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
     a[t * G + g] = val;
   }
   __syncthreads();
-  float2 *res = lds_fft<DIR>(a, b, pl, G, tw, tid, NT);
+  float2 *res = lds_fft<DIR, GEN>(a, b, pl, G, tw, tid, NT);
   for (int idx = tid; idx < n * G; idx += NT) {
     int t, g;
     coords(idx, t, g);
@@ -246,19 +259,6 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
 constexpr int PT = 1024;                 // threads of a plane block
 constexpr int PLANE_V = 16;              // values a thread holds across one pass
 constexpr size_t PLANE_LDS = 160 * 1024;  // LDS of one CU on gfx950
-// Division by d >= 1 as a multiply-high: q = floor(x / d) exactly for
-// 0 <= x < 2^16 and d <= 2^16 (m = floor((2^32 - 1) / d) + 1 exceeds 2^32 / d
-// by less than 1, so x m / 2^32 exceeds x / d by less than 2^-16 <= 1 / d).
-// d = 1 (m = 2^32) is taken apart.
-struct FastDiv {
-  uint32_t m;
-  int d;
-};
-__device__ __forceinline__ FastDiv fast_div(int d) { return {0xFFFFFFFFu / (uint32_t)d + 1u, d}; }
-__device__ __forceinline__ int div_of(int x, FastDiv f) {
-  return f.d == 1 ? x : (int)__umulhi((uint32_t)x, f.m);
-}
-
 template <int R, int DIR>
 __device__ __noinline__ void plane_pass(int poff, int twoff, int n, int npen, int es, int ps, int Ns, int tid) {
   extern __shared__ float2 smem[];
@@ -694,9 +694,11 @@ struct surfhip_ns_plan {
     touched.assign(nslabs, 0);
   }
 
-  size_t lds_rows_r2c() const { return sizeof(float2) * ((size_t)n1 + 2 * (size_t)n1 * (g_rows / 2)); }
+  // Rows kernels: twiddles, (c2r: the G half-spectrum rows,) the ping-pong
+  // pair and the G row offsets.
+  size_t lds_rows_r2c() const { return sizeof(float2) * ((size_t)n1 + 2 * (size_t)n1 * (g_rows / 2) + g_rows); }
   size_t lds_rows_c2r() const {
-    return sizeof(float2) * ((size_t)n1 + (size_t)g_rows * h1 + 2 * (size_t)n1 * (g_rows / 2));
+    return sizeof(float2) * ((size_t)n1 + (size_t)g_rows * h1 + 2 * (size_t)n1 * (g_rows / 2) + g_rows);
   }
   size_t lds_axis(int n) const { return sizeof(float2) * ((size_t)n + 2 * (size_t)n * g_axis); }
 
@@ -720,15 +722,15 @@ struct surfhip_ns_plan {
   // entry b at in + b in_bs / out + b out_bs with item item0 + b.
   void r2c(const float *x, int64_t xs, float2 *out, int nb = 1, int64_t x_bs = 0, int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
-    hipLaunchKernelGGL(k_rows_r2c, dim3(grid, nb), dim3(NT), lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1,
-                       h1, n2, n3, g_rows, f1, tw.p);
+    hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_r2c<true> : k_rows_r2c<false>, dim3(grid, nb), dim3(NT),
+                       lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1, h1, n2, n3, g_rows, f1, tw.p);
     SH_LAUNCHED();
   }
   void c2r(const float2 *in, float *out, int64_t os, float scale, int nb = 1, int64_t in_bs = 0,
            int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
-    hipLaunchKernelGGL(k_rows_c2r, dim3(grid, nb), dim3(NT), lds_rows_c2r(), stream, in, in_bs, out, os, out_bs,
-                       scale, n1, h1, n2, n3, g_rows, f1, tw.p);
+    hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_c2r<true> : k_rows_c2r<false>, dim3(grid, nb), dim3(NT),
+                       lds_rows_c2r(), stream, in, in_bs, out, os, out_bs, scale, n1, h1, n2, n3, g_rows, f1, tw.p);
     SH_LAUNCHED();
   }
   template <int AXIS, int DIR, int MODE>
@@ -738,8 +740,9 @@ struct surfhip_ns_plan {
     const FftPlan1D &pl = AXIS == 2 ? f2 : f3;
     if ((int64_t)pl.n * G > (int64_t)AXIS_MAXE * NT)
       fail(SURFHIP_ERR_ARG, fmt("axis block of %d x %d elements exceeds %d", pl.n, G, AXIS_MAXE * NT));
-    hipLaunchKernelGGL((k_axis<AXIS, DIR, MODE>), dim3(grid, nb), dim3(NT), lds_axis(pl.n), stream, in, in_bs, out,
-                       out_bs, h1, n2, n3, G, pl, tw.p, dview, ditems.p + item0);
+    auto *kernel = fft_plan_generic(pl) ? &k_axis<AXIS, DIR, MODE, true> : &k_axis<AXIS, DIR, MODE, false>;
+    hipLaunchKernelGGL(kernel, dim3(grid, nb), dim3(NT), lds_axis(pl.n), stream, in, in_bs, out, out_bs, h1, n2, n3,
+                       G, pl, tw.p, dview, ditems.p + item0);
     SH_LAUNCHED();
   }
   // FFT over axes 2 and 3 (DIR), modes as k_axis: the plane kernel where the
