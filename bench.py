@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--dims", type=lambda v: tuple(int(d) for d in v.split(",")), default=None,
+                    metavar="N1,N2,N3",
+                    help="ns workload on another volume size (default 128,128,128, the headline); sizes off "
+                         "the power-of-two grids run the general-size kernels")
     ap.add_argument("--workload", choices=("ns", "decimated"), default="ns",
                     help="ns: the NSSurfacelet headline (default); decimated: the decimated Surfacelet "
                          "(scales=3, direction_levels=[2,2,3], pyramid_ratio=2) on the same volume")
@@ -68,13 +72,13 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "read_bytes": n * 4, "write_bytes": n * 4}), flush=True)
 
 
-def cpu_baseline(x_np, budget_s):
+def cpu_baseline(x_np, budget_s, dims=DIMS):
     from oracle import ns as O
     from oracle.cpu_schedule import CPUSchedule
     workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     workers = max(1, min(workers, 16))
     t = O.NSParams(3, (2, 2, 3))
-    sch = CPUSchedule(t, DIMS, workers)
+    sch = CPUSchedule(t, dims, workers)
     S = sch.S
     out = [None] * S
     x = x_np.astype(np.float32)
@@ -88,9 +92,10 @@ def cpu_baseline(x_np, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = (time.perf_counter() - t0) / pairs
-    N = int(np.prod(DIMS))
+    N = int(np.prod(dims))
+    shape = "x".join(str(d) for d in dims)
     return {"value": N * S / dt, "unit": "voxel·subbands/s", "cores": workers, "kind": "port",
-            "sample": f"{pairs} analyze+synthesize pairs of the same 128^3 workload (FP32 scipy.fft, "
+            "sample": f"{pairs} analyze+synthesize pairs of the same {shape} workload (FP32 scipy.fft, "
                       f"{workers} worker threads, one single-threaded FFT per task), {dt * 1e3:.0f} ms per pair"}
 
 
@@ -209,13 +214,14 @@ def main():
     dev = "cuda"
     torch.cuda.set_device(0)
     t = NS.NSSurfacelet(scales=3, direction_levels=[2, 2, 3])
-    x_np = np.random.default_rng(SEED).standard_normal(DIMS).astype(np.float32)
+    dims = args.dims or DIMS
+    x_np = np.random.default_rng(SEED).standard_normal(dims).astype(np.float32)
     x = from_numpy(x_np, dev)
     ws = NS.make_workspace(t, x)
-    S = NS.nsubbands(t, DIMS)
+    S = NS.nsubbands(t, dims)
     c = NS.similar_coefficients(t, x)
     y = torch.empty_like(x)
-    N = int(np.prod(DIMS))
+    N = int(np.prod(dims))
 
     def step():
         NS.analyze_(c, ws, x)
@@ -242,10 +248,14 @@ def main():
     cls = ws.timing()
     ws.set_timing(False)
     kernels = {k: {"launches_per_step": n / args.steps, "ms_per_step": t / args.steps} for k, (n, t) in cls.items()}
-    # Algorithmic HBM bytes per step of the two plane kernels, the ones that
-    # touch the user's data: the inverse plane pass writes the S subbands and
-    # the volume, the forward plane pass reads the volume and the S subbands.
-    alg = {"plane_inverse": 4 * N * (S + 1), "plane_forward": 4 * N * (S + 1)}
+    # Algorithmic HBM bytes per step of the two kernel classes that touch the
+    # user's data: the one that writes the S subbands and the volume (factored
+    # path: the inverse plane pass; general path: the axis-1 c2r) and the one
+    # that reads the volume and the S subbands (forward plane pass; axis-1 r2c).
+    if "plane_inverse" in cls:
+        alg = {"plane_inverse": 4 * N * (S + 1), "plane_forward": 4 * N * (S + 1)}
+    else:
+        alg = {"rows_c2r": 4 * N * (S + 1), "rows_r2c": 4 * N * (S + 1)}
     dom = max(cls, key=lambda k: cls[k][1])
     dom_plane = max(alg, key=lambda k: cls.get(k, (0, 0.0))[1])
     launches, tms = cls[dom_plane]
@@ -253,8 +263,11 @@ def main():
 
     bytes_per_step = 8 * N * (S + 1)
     achieved = bytes_per_step / (ms * 1e-3) / 1e9
+    headline = tuple(dims) == DIMS
+    shape = "x".join(str(d) for d in dims)
     result = {
-        "metric": "analyze+synth voxel·subbands/s, 128³ FP32, 49 subbands; % HBM peak",
+        "metric": ("analyze+synth voxel·subbands/s, 128³ FP32, 49 subbands; % HBM peak" if headline else
+                   f"analyze+synth voxel·subbands/s, {shape} FP32, {S} subbands; % HBM peak"),
         "value": N * S / (ms * 1e-3),
         "unit": "voxel·subbands/s",
         "n_gpus": 1,
@@ -267,10 +280,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic: i.i.d. N(0,1) Float32 volume, numpy PCG64 seed 20260",
         "config": {"workload": "NSSurfacelet(scales=3, direction_levels=[2,2,3]) analyze!+synthesize!, "
-                               "128x128x128 Float32, 49 subbands, prebuilt workspace",
-                   "dims": list(DIMS), "subbands": S, "parallelism": "single GPU"},
+                               f"{shape} Float32, {S} subbands, prebuilt workspace",
+                   "dims": list(dims), "subbands": S, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": dom_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (dom_achieved / HBM_PEAK_GBS) if dom_achieved else None, "traffic": TRAFFIC.get(dom_plane),
+                     "frac": (dom_achieved / HBM_PEAK_GBS) if dom_achieved else None,
+                     "traffic": TRAFFIC.get(dom_plane) if headline else None,
                      "kernel": dom_plane, "launches_per_step": launches / args.steps,
                      "avg_launch_us": tms / launches * 1e3 if launches else None,
                      "dominant_class": dom},
@@ -280,7 +294,7 @@ def main():
         "check": {"reconstruction_rel_l2": rel},
     }
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(x_np, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(x_np, args.cpu_seconds, dims)
     print(json.dumps(result), flush=True)
 
 

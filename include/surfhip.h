@@ -191,9 +191,11 @@ int surfhip_ns_group_table_host(const surfhip_ns_params *p, int32_t n1, int32_t 
  * by HIP events on the plan's stream.  surfhip_ns_timing synchronises the
  * stream and returns, per kernel class, the number of launches and their
  * summed duration in milliseconds since timing was last switched on; class
- * names are written comma-separated into `names`.  Classes: 0 pencil
- * analysis, 1 plane analysis (subband write), 2 plane synthesis (subband
- * read), 3 pencil synthesis, 4 other. */
+ * names are written comma-separated into `names`.  Classes of the factored
+ * path: 0 pencil analysis, 1 plane analysis (subband write), 2 plane
+ * synthesis (subband read), 3 pencil synthesis, 4 other.  Of the general
+ * path: 0 axis-1 r2c (volume and subband read), 1 axis-1 c2r (subband and
+ * volume write), 2 axes 2-3 forward, 3 axes 2-3 inverse, 4 other. */
 #define SURFHIP_NS_TIMING_CLASSES 5
 int surfhip_ns_set_timing(surfhip_ns_plan *plan, int32_t enable);
 int surfhip_ns_timing(surfhip_ns_plan *plan, int64_t *launches, double *ms, char *names, size_t n);

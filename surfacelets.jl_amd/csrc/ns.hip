@@ -331,6 +331,52 @@ __device__ void plane_fft(int poff, int twoff, const FftPlan1D &pl, int npen, in
   }
 }
 
+// The mask of one subband over the (k2, k3) plane of one k1: the group
+// table row k1 and each wedge plane as a base and (k2, k3) strides, so an
+// element costs three loads.  Same values, and the same product
+// G (w_a w_b), as ns_mask.
+struct PlaneMask {
+  const float *g, *wa, *wb;  // wa, wb null: factor 1
+  int a2, a3, b2, b3;
+  bool unit;
+};
+__device__ __forceinline__ const float *plane_wedge(const NSMaskView &v, int32_t off, int32_t pair, int k1, int &s2,
+                                                    int &s3) {
+  if (off < 0) return nullptr;
+  const float *base = v.planes + off;
+  if (pair == 0) {  // (k1, k2)
+    s2 = 1;
+    s3 = 0;
+    return base + (size_t)k1 * v.n2;
+  }
+  if (pair == 1) {  // (k1, k3)
+    s2 = 0;
+    s3 = 1;
+    return base + (size_t)k1 * v.n3;
+  }
+  s2 = v.n3;  // (k2, k3)
+  s3 = 1;
+  return base;
+}
+__device__ __forceinline__ PlaneMask plane_mask(const NSMaskView &v, const NSItem &it, int k1, bool analysis) {
+  PlaneMask m{};
+  m.g = ns_group_table(v, ns_group_index(v, it), analysis) + (size_t)k1 * v.n2 * v.n3;
+  m.unit = it.lowpass || ns_unit_wedge(v, analysis);
+  if (!m.unit) {
+    m.wa = plane_wedge(v, it.pa_off, it.pa_pair, k1, m.a2, m.a3);
+    m.wb = plane_wedge(v, it.pb_off, it.pb_pair, k1, m.b2, m.b3);
+  }
+  return m;
+}
+// Mask at plane element i = k2 n3 + k3.
+__device__ __forceinline__ float plane_mask_at(const PlaneMask &m, int i, int k2, int k3) {
+  const float g = m.g[i];
+  if (m.unit) return g * 1.0f;
+  const float wa = m.wa ? m.wa[k2 * m.a2 + k3 * m.a3] : 1.0f;
+  const float wb = m.wb ? m.wb[k2 * m.b2 + k3 * m.b3] : 1.0f;
+  return g * (wa * wb);
+}
+
 // 2-D FFT over the (k2, k3) plane of one k1 (blockIdx.x) of a plane-major
 // spectrum, the plane held in LDS with row pitch n3p (odd, so that strided
 // pencils fall in distinct banks).  Modes as k_axis: 0 plain, 1 mask before
@@ -354,12 +400,12 @@ __global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_plane(const float2 *in, int
   const int k1 = blockIdx.x;
   const int nel = n2 * n3;
   const int64_t base = (int64_t)k1 * nel;
-  const NSItem it = items[blockIdx.y];
+  const PlaneMask pm = plane_mask(mv, items[blockIdx.y], k1, MODE == 1);
   const FastDiv d3 = fast_div(n3);
   for (int i = tid; i < nel; i += PT) {
     const int k2 = div_of(i, d3), k3 = i - k2 * n3;
     float2 v = in[base + i];
-    if (MODE == 1) v = cscale(v, ns_mask(mv, it, k1, k2, k3, true));
+    if (MODE == 1) v = cscale(v, plane_mask_at(pm, i, k2, k3));
     P[k2 * n3p + k3] = v;
   }
   __syncthreads();
@@ -368,7 +414,7 @@ __global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_plane(const float2 *in, int
   for (int i = tid; i < nel; i += PT) {
     const int k2 = div_of(i, d3), k3 = i - k2 * n3;
     float2 v = P[k2 * n3p + k3];
-    if (MODE == 2) v = cscale(v, ns_mask(mv, it, k1, k2, k3, false));
+    if (MODE == 2) v = cscale(v, plane_mask_at(pm, i, k2, k3));
     out[base + i] = v;
   }
 }
@@ -722,15 +768,19 @@ struct surfhip_ns_plan {
   // entry b at in + b in_bs / out + b out_bs with item item0 + b.
   void r2c(const float *x, int64_t xs, float2 *out, int nb = 1, int64_t x_bs = 0, int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
-    hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_r2c<true> : k_rows_r2c<false>, dim3(grid, nb), dim3(NT),
-                       lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1, h1, n2, n3, g_rows, f1, tw.p);
+    timed(0, [&] {
+      hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_r2c<true> : k_rows_r2c<false>, dim3(grid, nb), dim3(NT),
+                         lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1, h1, n2, n3, g_rows, f1, tw.p);
+    });
     SH_LAUNCHED();
   }
   void c2r(const float2 *in, float *out, int64_t os, float scale, int nb = 1, int64_t in_bs = 0,
            int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
-    hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_c2r<true> : k_rows_c2r<false>, dim3(grid, nb), dim3(NT),
-                       lds_rows_c2r(), stream, in, in_bs, out, os, out_bs, scale, n1, h1, n2, n3, g_rows, f1, tw.p);
+    timed(1, [&] {
+      hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_c2r<true> : k_rows_c2r<false>, dim3(grid, nb), dim3(NT),
+                         lds_rows_c2r(), stream, in, in_bs, out, os, out_bs, scale, n1, h1, n2, n3, g_rows, f1, tw.p);
+    });
     SH_LAUNCHED();
   }
   template <int AXIS, int DIR, int MODE>
@@ -741,8 +791,10 @@ struct surfhip_ns_plan {
     if ((int64_t)pl.n * G > (int64_t)AXIS_MAXE * NT)
       fail(SURFHIP_ERR_ARG, fmt("axis block of %d x %d elements exceeds %d", pl.n, G, AXIS_MAXE * NT));
     auto *kernel = fft_plan_generic(pl) ? &k_axis<AXIS, DIR, MODE, true> : &k_axis<AXIS, DIR, MODE, false>;
-    hipLaunchKernelGGL(kernel, dim3(grid, nb), dim3(NT), lds_axis(pl.n), stream, in, in_bs, out, out_bs, h1, n2, n3,
-                       G, pl, tw.p, dview, ditems.p + item0);
+    timed(DIR < 0 ? 2 : 3, [&] {
+      hipLaunchKernelGGL(kernel, dim3(grid, nb), dim3(NT), lds_axis(pl.n), stream, in, in_bs, out, out_bs, h1, n2,
+                         n3, G, pl, tw.p, dview, ditems.p + item0);
+    });
     SH_LAUNCHED();
   }
   // FFT over axes 2 and 3 (DIR), modes as k_axis: the plane kernel where the
@@ -751,12 +803,11 @@ struct surfhip_ns_plan {
   template <int DIR, int MODE>
   void axes23(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
     if (plane) {
-      if (plane_two())
-        hipLaunchKernelGGL((k_plane<DIR, MODE, true>), dim3(h1, nb), dim3(PT), lds_plane(), stream, in, in_bs, out,
-                           out_bs, n2, n3, n3p, f2, f3, tw.p, dview, ditems.p + item0);
-      else
-        hipLaunchKernelGGL((k_plane<DIR, MODE, false>), dim3(h1, nb), dim3(PT), lds_plane(), stream, in, in_bs, out,
-                           out_bs, n2, n3, n3p, f2, f3, tw.p, dview, ditems.p + item0);
+      auto *kernel = plane_two() ? &k_plane<DIR, MODE, true> : &k_plane<DIR, MODE, false>;
+      timed(DIR < 0 ? 2 : 3, [&] {
+        hipLaunchKernelGGL(kernel, dim3(h1, nb), dim3(PT), lds_plane(), stream, in, in_bs, out, out_bs, n2, n3, n3p,
+                           f2, f3, tw.p, dview, ditems.p + item0);
+      });
       SH_LAUNCHED();
       return;
     }
@@ -766,7 +817,7 @@ struct surfhip_ns_plan {
   // acc += work_0 + ... + work_{nb-1} in order.
   void sum_into(float2 *a, const float2 *w, int nb) {
     const int64_t blocks = std::min<int64_t>((H + NT - 1) / NT, 4096);
-    hipLaunchKernelGGL(k_sum, dim3((unsigned)blocks), dim3(NT), 0, stream, a, w, H, nb, H);
+    timed(4, [&] { hipLaunchKernelGGL(k_sum, dim3((unsigned)blocks), dim3(NT), 0, stream, a, w, H, nb, H); });
     SH_LAUNCHED();
   }
   void zero(float2 *p, int64_t n) {
@@ -1373,7 +1424,7 @@ int surfhip_ns_timing(surfhip_ns_plan *plan, int64_t *launches, double *ms, char
       if (ms) ms[c] = plan->tms[c];
     }
     const char *s = plan->fused ? "pencil_analysis,plane_inverse,plane_forward,pencil_synthesis,other"
-                               : "unclassified,unclassified,unclassified,unclassified,unclassified";
+                               : "rows_r2c,rows_c2r,axes23_forward,axes23_inverse,other";
     if (names && n) {
       size_t m = std::min(n - 1, std::strlen(s));
       std::memcpy(names, s, m);
