@@ -5,7 +5,8 @@
 // reads a[j + r n/R], applies the twiddles w^(r (j mod Ns) n/(Ns R)), runs an
 // R-point DFT in registers and writes b[(j/Ns) Ns R + (j mod Ns) + r Ns]
 // (decimation in time, self-sorting: no bit reversal), ping-ponging two LDS
-// buffers.  Radices 8, 4, 2, 3, 5, 7 are hand-written; other primes up to
+// buffers.  Radices 2, 3, 4, 5, 7, 8 are hand-written, and 6, 9, 10, 12, 14,
+// 16 are two-level composites of them in registers; other primes up to
 // FFT_MAX_RADIX use a direct DFT.  Twiddles w^m = exp(-2 pi i m / n) are made
 // in FP64 on the host and rounded once to FP32.
 #pragma once
@@ -27,8 +28,9 @@ struct FftPlan1D {
   int32_t radix[FFT_MAX_PASSES];
 };
 
-// Host: factor n (radix 8 first, then 4, 2, 3, 5, 7 and remaining primes) and
-// append its twiddles to `pool`.  Throws for a prime factor > FFT_MAX_RADIX.
+// Host: factor n (the hand-written radices, largest first, then the remaining
+// primes) and append its twiddles to `pool`.  Throws for a prime factor >
+// FFT_MAX_RADIX.
 FftPlan1D make_fft_plan(int n, std::vector<float2> &pool);
 
 // Division by d >= 1 as a multiply-high: q = floor(x / d) exactly for
@@ -151,6 +153,129 @@ template <int DIR> __device__ __forceinline__ void dft7(float2 *v) {
 }
 
 // Direct R-point DFT for other primes, twiddles from the length-n table.
+// cos and sin of 2 pi m / R for the composite radices, FP64 values rounded
+// once to FP32 (m = 0 .. R-1).
+__host__ __device__ constexpr float unit_cos(int R, int m) {
+  switch (R) {
+    case 6: {
+      constexpr float t[6] = {1.00000000000000000000f, 0.50000000000000011102f, -0.49999999999999977796f, -1.00000000000000000000f, -0.50000000000000044409f, 0.50000000000000011102f};
+      return t[m];
+    }
+    case 9: {
+      constexpr float t[9] = {1.00000000000000000000f, 0.76604444311897801345f, 0.17364817766693041445f, -0.49999999999999977796f, -0.93969262078590831688f, -0.93969262078590842791f, -0.50000000000000044409f, 0.17364817766692997036f, 0.76604444311897779141f};
+      return t[m];
+    }
+    case 10: {
+      constexpr float t[10] = {1.00000000000000000000f, 0.80901699437494745126f, 0.30901699437494745126f, -0.30901699437494734024f, -0.80901699437494734024f, -1.00000000000000000000f, -0.80901699437494756229f, -0.30901699437494756229f, 0.30901699437494722922f, 0.80901699437494734024f};
+      return t[m];
+    }
+    case 12: {
+      constexpr float t[12] = {1.00000000000000000000f, 0.86602540378443870761f, 0.50000000000000011102f, 0.00000000000000006123f, -0.49999999999999977796f, -0.86602540378443870761f, -1.00000000000000000000f, -0.86602540378443881863f, -0.50000000000000044409f, -0.00000000000000018370f, 0.50000000000000011102f, 0.86602540378443837454f};
+      return t[m];
+    }
+    case 14: {
+      constexpr float t[14] = {1.00000000000000000000f, 0.90096886790241914600f, 0.62348980185873359439f, 0.22252093395631444839f, -0.22252093395631433737f, -0.62348980185873348336f, -0.90096886790241903498f, -1.00000000000000000000f, -0.90096886790241914600f, -0.62348980185873370541f, -0.22252093395631458717f, 0.22252093395631333816f, 0.62348980185873337234f, 0.90096886790241936804f};
+      return t[m];
+    }
+    case 16: {
+      constexpr float t[16] = {1.00000000000000000000f, 0.92387953251128673848f, 0.70710678118654757274f, 0.38268343236508983729f, 0.00000000000000006123f, -0.38268343236508972627f, -0.70710678118654746172f, -0.92387953251128673848f, -1.00000000000000000000f, -0.92387953251128684951f, -0.70710678118654768376f, -0.38268343236509033689f, -0.00000000000000018370f, 0.38268343236509000382f, 0.70710678118654735069f, 0.92387953251128651644f};
+      return t[m];
+    }
+    default: return 0.0f;
+  }
+}
+__host__ __device__ constexpr float unit_sin(int R, int m) {
+  switch (R) {
+    case 6: {
+      constexpr float t[6] = {0.00000000000000000000f, 0.86602540378443859659f, 0.86602540378443870761f, 0.00000000000000012246f, -0.86602540378443837454f, -0.86602540378443859659f};
+      return t[m];
+    }
+    case 9: {
+      constexpr float t[9] = {0.00000000000000000000f, 0.64278760968653925190f, 0.98480775301220802032f, 0.86602540378443870761f, 0.34202014332566887944f, -0.34202014332566865740f, -0.86602540378443837454f, -0.98480775301220813134f, -0.64278760968653958496f};
+      return t[m];
+    }
+    case 10: {
+      constexpr float t[10] = {0.00000000000000000000f, 0.58778525229247313710f, 0.95105651629515353118f, 0.95105651629515364220f, 0.58778525229247324813f, 0.00000000000000012246f, -0.58778525229247302608f, -0.95105651629515353118f, -0.95105651629515364220f, -0.58778525229247335915f};
+      return t[m];
+    }
+    case 12: {
+      constexpr float t[12] = {0.00000000000000000000f, 0.49999999999999994449f, 0.86602540378443859659f, 1.00000000000000000000f, 0.86602540378443870761f, 0.49999999999999994449f, 0.00000000000000012246f, -0.49999999999999972244f, -0.86602540378443837454f, -1.00000000000000000000f, -0.86602540378443859659f, -0.50000000000000044409f};
+      return t[m];
+    }
+    case 14: {
+      constexpr float t[14] = {0.00000000000000000000f, 0.43388373911755812040f, 0.78183148246802980363f, 0.97492791218182361934f, 0.97492791218182361934f, 0.78183148246802991466f, 0.43388373911755823142f, 0.00000000000000012246f, -0.43388373911755800938f, -0.78183148246802969261f, -0.97492791218182361934f, -0.97492791218182384139f, -0.78183148246802991466f, -0.43388373911755750978f};
+      return t[m];
+    }
+    case 16: {
+      constexpr float t[16] = {0.00000000000000000000f, 0.38268343236508978178f, 0.70710678118654746172f, 0.92387953251128673848f, 1.00000000000000000000f, 0.92387953251128673848f, 0.70710678118654757274f, 0.38268343236508989280f, 0.00000000000000012246f, -0.38268343236508967076f, -0.70710678118654746172f, -0.92387953251128651644f, -1.00000000000000000000f, -0.92387953251128662746f, -0.70710678118654768376f, -0.38268343236509039240f};
+      return t[m];
+    }
+    default: return 0.0f;
+  }
+}
+
+// a exp(DIR 2 pi i m / R), m a constant once unrolled: quarter and half turns
+// exactly, the rest with the tabulated cos and sin.
+template <int R, int DIR> __device__ __forceinline__ float2 root_mul(float2 a, int m) {
+  m %= R;
+  if (m == 0) return a;
+  if (4 * m == R) return rot90<DIR>(a);
+  if (2 * m == R) return make_float2(-a.x, -a.y);
+  if (4 * m == 3 * R) return rot90<-DIR>(a);
+  const float c = unit_cos(R, m), s = DIR < 0 ? -unit_sin(R, m) : unit_sin(R, m);
+  return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+template <int N, int DIR> __device__ __forceinline__ void dft_radix(float2 *v);
+
+// Composite radix R = P Q (Cooley-Tukey in registers): input r = q + Q p,
+// output k = k1 + P k2.  P-point DFTs over p for each q, the internal twiddles
+// w_R^(q k1), then Q-point DFTs over q for each k1.
+template <int P, int Q, int DIR> __device__ __forceinline__ void dft_pq(float2 *v) {
+  constexpr int R = P * Q;
+  float2 y[Q][P];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) y[q][p] = v[q + Q * p];
+    dft_radix<P, DIR>(y[q]);
+#pragma unroll
+    for (int k1 = 1; k1 < P; ++k1) y[q][k1] = root_mul<R, DIR>(y[q][k1], q * k1);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) {
+    float2 z[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) z[q] = y[q][k1];
+    dft_radix<Q, DIR>(z);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = z[k2];
+  }
+}
+
+// The hand-written butterflies: 2, 3, 4, 5, 7, 8 directly; 6, 9, 10, 12, 14
+// and 16 as two-level composites of them.
+template <int N, int DIR> __device__ __forceinline__ void dft_radix(float2 *v) {
+  if constexpr (N == 2) dft2<DIR>(v);
+  else if constexpr (N == 3) dft3<DIR>(v);
+  else if constexpr (N == 4) dft4<DIR>(v);
+  else if constexpr (N == 5) dft5<DIR>(v);
+  else if constexpr (N == 7) dft7<DIR>(v);
+  else if constexpr (N == 8) dft8<DIR>(v);
+  else if constexpr (N == 6) dft_pq<2, 3, DIR>(v);
+  else if constexpr (N == 9) dft_pq<3, 3, DIR>(v);
+  else if constexpr (N == 10) dft_pq<2, 5, DIR>(v);
+  else if constexpr (N == 12) dft_pq<4, 3, DIR>(v);
+  else if constexpr (N == 14) dft_pq<2, 7, DIR>(v);
+  else if constexpr (N == 16) dft_pq<4, 4, DIR>(v);
+}
+
+// Whether radix R has a hand-written butterfly.
+__host__ __device__ constexpr bool fft_radix_fixed(int R) {
+  return R == 2 || R == 3 || R == 4 || R == 5 || R == 6 || R == 7 || R == 8 || R == 9 || R == 10 || R == 12 ||
+         R == 14 || R == 16;
+}
+
 template <int DIR> __device__ __noinline__ void dft_generic(float2 *v, int R, const float2 *tw, int n) {
   float2 out[FFT_MAX_RADIX];
   const int step = n / R;
@@ -183,12 +308,7 @@ template <int R, int DIR> __device__ __forceinline__ void fft_pass_fixed(
 #pragma unroll
       for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<DIR>(tw, r * k * tstep));
     }
-    if (R == 2) dft2<DIR>(v);
-    else if (R == 3) dft3<DIR>(v);
-    else if (R == 4) dft4<DIR>(v);
-    else if (R == 5) dft5<DIR>(v);
-    else if (R == 7) dft7<DIR>(v);
-    else if (R == 8) dft8<DIR>(v);
+    dft_radix<R, DIR>(v);
     const int idxD = jn * Ns * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) b[(idxD + r * Ns) * G + g] = v[r];
@@ -215,10 +335,8 @@ template <int DIR> __device__ __noinline__ void fft_pass_generic(
 // Whether a plan has a radix without a hand-written butterfly (it then
 // needs lds_fft's direct-DFT pass).
 inline bool fft_plan_generic(const FftPlan1D &pl) {
-  for (int p = 0; p < pl.npass; ++p) {
-    const int R = pl.radix[p];
-    if (R != 2 && R != 3 && R != 4 && R != 5 && R != 7 && R != 8) return true;
-  }
+  for (int p = 0; p < pl.npass; ++p)
+    if (!fft_radix_fixed(pl.radix[p])) return true;
   return false;
 }
 
@@ -244,6 +362,12 @@ __device__ float2 *lds_fft(float2 *a, float2 *b, const FftPlan1D &pl, int G, con
       case 3: fft_pass_fixed<3, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
       case 5: fft_pass_fixed<5, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
       case 7: fft_pass_fixed<7, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 16: fft_pass_fixed<16, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 14: fft_pass_fixed<14, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 12: fft_pass_fixed<12, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 10: fft_pass_fixed<10, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 9: fft_pass_fixed<9, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
+      case 6: fft_pass_fixed<6, DIR>(a, b, n, G, dG, Ns, dNs, tw, tid, nthreads); break;
       default:
         if (GEN) fft_pass_generic<DIR>(a, b, n, R, G, Ns, tw, tid, nthreads);
         break;

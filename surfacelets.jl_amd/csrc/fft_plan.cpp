@@ -15,14 +15,16 @@ FftPlan1D make_fft_plan(int n, std::vector<float2> &pool) {
   pl.n = n;
   pl.npass = 0;
   int m = n;
-  const int fixed[] = {8, 4, 2, 3, 5, 7};
+  // Largest hand-written radices first: fewer passes (96 = 16 x 6,
+  // 80 = 16 x 5, 72 = 12 x 6, 112 = 16 x 7, 128 = 16 x 8).
+  const int fixed[] = {16, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
   for (int r : fixed)
     while (m % r == 0) {
       if (pl.npass >= FFT_MAX_PASSES) fail(SURFHIP_ERR_ARG, fmt("FFT length %d has too many factors", n));
       pl.radix[pl.npass++] = r;
       m /= r;
     }
-  for (int p = 7; m > 1; p += 2) {
+  for (int p = 11; m > 1; p += 2) {
     while (m % p == 0) {
       if (p > FFT_MAX_RADIX)
         fail(SURFHIP_ERR_ARG, fmt("axis length %d has the prime factor %d; factors up to %d are supported", n, p, FFT_MAX_RADIX));

@@ -293,12 +293,7 @@ __device__ __noinline__ void plane_pass(int poff, int twoff, int n, int npen, in
 #pragma unroll
         for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twid<DIR>(tw, r * k * tstep));
       }
-      if (R == 2) dft2<DIR>(v[q]);
-      else if (R == 3) dft3<DIR>(v[q]);
-      else if (R == 4) dft4<DIR>(v[q]);
-      else if (R == 5) dft5<DIR>(v[q]);
-      else if (R == 7) dft7<DIR>(v[q]);
-      else if (R == 8) dft8<DIR>(v[q]);
+      dft_radix<R, DIR>(v[q]);
       // output index jn Ns R + k = j + jn Ns (R - 1)
       const int o = at[q] + jn * Ns * (R - 1) * es;
 #pragma unroll
@@ -310,8 +305,7 @@ __device__ __noinline__ void plane_pass(int poff, int twoff, int n, int npen, in
 
 // Whether radix R fits the plane pass for npen pencils of length n.
 inline bool plane_radix_fits(int R, int64_t elems) {
-  const bool known = R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8;
-  return known && (elems / R + PT - 1) / PT <= PLANE_V / R;
+  return fft_radix_fixed(R) && (elems / R + PT - 1) / PT <= PLANE_V / R;
 }
 
 template <int DIR>
@@ -325,7 +319,13 @@ __device__ void plane_fft(int poff, int twoff, const FftPlan1D &pl, int npen, in
       case 2: plane_pass<2, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
       case 3: plane_pass<3, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
       case 5: plane_pass<5, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      default: plane_pass<7, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 7: plane_pass<7, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 16: plane_pass<16, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 14: plane_pass<14, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 12: plane_pass<12, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 10: plane_pass<10, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 9: plane_pass<9, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      default: plane_pass<6, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
     }
     Ns *= R;
   }
