@@ -1214,9 +1214,11 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       P->make_aux();
     } else {
       for (int ch = 0; ch < P->nch; ++ch) P->acc[ch].alloc(P->H);
-      // Batched general-path launches: up to nwork subbands per launch, as
-      // long as their work buffers stay within 8 volumes of the spectrum.
-      P->nwork = 8;
+      // Batched general-path launches: up to nwork subbands per launch, their
+      // work spectra within 1 GiB (at least 8 of them).  Large batches make
+      // launches of many blocks, so the last wave of blocks costs little.
+      const int64_t fit = ((int64_t)1 << 30) / ((int64_t)sizeof(float2) * P->H);
+      P->nwork = (int)std::max<int64_t>(1, std::min<int64_t>(P->T.S - 1, std::max<int64_t>(8, fit)));
       P->work.alloc((size_t)P->H * P->nwork);
     }
     // The group-table inputs are no longer needed on the host.
