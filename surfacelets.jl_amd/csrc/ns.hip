@@ -50,7 +50,7 @@ template <bool GEN>
 __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, int64_t xs, int64_t x_bs,
                                                  float2 *__restrict__ out, int64_t out_bs, int n1, int h1,
                                                  int n2, int n3, int G, FftPlan1D pl,
-                                                 const float2 *__restrict__ twg) {
+                                                 const float2 *__restrict__ twg, int vec) {
   // Batch entry blockIdx.y: its own input and output.
   x += blockIdx.y * x_bs;
   out += blockIdx.y * out_bs;
@@ -67,12 +67,28 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, in
   real_rows(roff, r0, G, n2, n3, n1, tid);
   __syncthreads();
   const FastDiv dn1 = fast_div(n1), dG = fast_div(G);
-  for (int idx = tid; idx < G * n1; idx += NT) {
-    const int g = div_of(idx, dn1), t = idx - g * n1;
-    const int rp = r0 + g;
-    float v = rp < nrows ? x[(roff[g] + t) * xs] : 0.0f;
-    float *ap = reinterpret_cast<float *>(&a[t * P + (g >> 1)]);
-    ap[g & 1] = v;
+  if (vec) {
+    // Unit-stride rows, n1 a multiple of 4, 16-byte aligned: 16-byte loads.
+    const int n4 = n1 >> 2;
+    const FastDiv dn4 = fast_div(n4);
+    for (int idx = tid; idx < G * n4; idx += NT) {
+      const int g = div_of(idx, dn4), t = 4 * (idx - g * n4);
+      const int rp = r0 + g;
+      const float4 v = rp < nrows ? *reinterpret_cast<const float4 *>(x + roff[g] + t) : make_float4(0.f, 0.f, 0.f, 0.f);
+      float *ap = reinterpret_cast<float *>(&a[t * P + (g >> 1)]) + (g & 1);
+      ap[0] = v.x;
+      ap[2 * P] = v.y;
+      ap[4 * P] = v.z;
+      ap[6 * P] = v.w;
+    }
+  } else {
+    for (int idx = tid; idx < G * n1; idx += NT) {
+      const int g = div_of(idx, dn1), t = idx - g * n1;
+      const int rp = r0 + g;
+      float v = rp < nrows ? x[(roff[g] + t) * xs] : 0.0f;
+      float *ap = reinterpret_cast<float *>(&a[t * P + (g >> 1)]);
+      ap[g & 1] = v;
+    }
   }
   __syncthreads();
   float2 *res = lds_fft<-1, GEN>(a, b, pl, P, tw, tid, NT);
@@ -102,7 +118,7 @@ template <bool GEN>
 __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, int64_t in_bs,
                                                  float *__restrict__ out, int64_t os, int64_t out_bs, float scale,
                                                  int n1, int h1, int n2, int n3, int G, FftPlan1D pl,
-                                                 const float2 *__restrict__ twg) {
+                                                 const float2 *__restrict__ twg, int vec) {
   in += blockIdx.y * in_bs;
   out += blockIdx.y * out_bs;
   extern __shared__ float2 smem[];
@@ -118,10 +134,22 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
   const int r0 = blockIdx.x * G;
   real_rows(roff, r0, G, n2, n3, n1, tid);
   const FastDiv dn1 = fast_div(n1), dG = fast_div(G), dP = fast_div(P);
-  for (int idx = tid; idx < G * h1; idx += NT) {
-    const int k = div_of(idx, dG), g = idx - k * G;
-    const int rp = r0 + g;
-    s[g * h1 + k] = rp < nrows ? in[(int64_t)k * nrows + rp] : make_float2(0.0f, 0.0f);
+  if (vec & 1) {
+    // Even row count and 16-byte aligned bins: two rows' bins per load.
+    for (int idx = tid; idx < P * h1; idx += NT) {
+      const int k = div_of(idx, dP), g = 2 * (idx - k * P);
+      const int rp = r0 + g;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (rp < nrows) v = *reinterpret_cast<const float4 *>(in + (int64_t)k * nrows + rp);
+      s[g * h1 + k] = make_float2(v.x, v.y);
+      s[(g + 1) * h1 + k] = make_float2(v.z, v.w);
+    }
+  } else {
+    for (int idx = tid; idx < G * h1; idx += NT) {
+      const int k = div_of(idx, dG), g = idx - k * G;
+      const int rp = r0 + g;
+      s[g * h1 + k] = rp < nrows ? in[(int64_t)k * nrows + rp] : make_float2(0.0f, 0.0f);
+    }
   }
   __syncthreads();
   const bool even = (n1 & 1) == 0;
@@ -143,12 +171,26 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
   }
   __syncthreads();
   float2 *res = lds_fft<1, GEN>(a, b, pl, P, tw, tid, NT);
-  for (int idx = tid; idx < G * n1; idx += NT) {
-    const int g = div_of(idx, dn1), t = idx - g * n1;
-    const int rp = r0 + g;
-    if (rp >= nrows) continue;
-    const float2 v = res[t * P + (g >> 1)];
-    out[(roff[g] + t) * os] = ((g & 1) ? v.y : v.x) * scale;
+  if (vec & 2) {
+    // Unit-stride rows, n1 a multiple of 4, 16-byte aligned: 16-byte stores.
+    const int n4 = n1 >> 2;
+    const FastDiv dn4 = fast_div(n4);
+    for (int idx = tid; idx < G * n4; idx += NT) {
+      const int g = div_of(idx, dn4), t = 4 * (idx - g * n4);
+      const int rp = r0 + g;
+      if (rp >= nrows) continue;
+      const float *rv = reinterpret_cast<const float *>(&res[t * P + (g >> 1)]) + (g & 1);
+      *reinterpret_cast<float4 *>(out + roff[g] + t) =
+          make_float4(rv[0] * scale, rv[2 * P] * scale, rv[4 * P] * scale, rv[6 * P] * scale);
+    }
+  } else {
+    for (int idx = tid; idx < G * n1; idx += NT) {
+      const int g = div_of(idx, dn1), t = idx - g * n1;
+      const int rp = r0 + g;
+      if (rp >= nrows) continue;
+      const float2 v = res[t * P + (g >> 1)];
+      out[(roff[g] + t) * os] = ((g & 1) ? v.y : v.x) * scale;
+    }
   }
 }
 
@@ -768,18 +810,22 @@ struct surfhip_ns_plan {
   // entry b at in + b in_bs / out + b out_bs with item item0 + b.
   void r2c(const float *x, int64_t xs, float2 *out, int nb = 1, int64_t x_bs = 0, int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
+    const int vec = xs == 1 && n1 % 4 == 0 && x_bs % 4 == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
     timed(0, [&] {
       hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_r2c<true> : k_rows_r2c<false>, dim3(grid, nb), dim3(NT),
-                         lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1, h1, n2, n3, g_rows, f1, tw.p);
+                         lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1, h1, n2, n3, g_rows, f1, tw.p, vec);
     });
     SH_LAUNCHED();
   }
   void c2r(const float2 *in, float *out, int64_t os, float scale, int nb = 1, int64_t in_bs = 0,
            int64_t out_bs = 0) {
     int grid = (rows() + g_rows - 1) / g_rows;
+    const int vin = rows() % 2 == 0 && in_bs % 2 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    const int vout = os == 1 && n1 % 4 == 0 && out_bs % 4 == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
     timed(1, [&] {
       hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_c2r<true> : k_rows_c2r<false>, dim3(grid, nb), dim3(NT),
-                         lds_rows_c2r(), stream, in, in_bs, out, os, out_bs, scale, n1, h1, n2, n3, g_rows, f1, tw.p);
+                         lds_rows_c2r(), stream, in, in_bs, out, os, out_bs, scale, n1, h1, n2, n3, g_rows, f1, tw.p,
+                         vin | (vout << 1));
     });
     SH_LAUNCHED();
   }
