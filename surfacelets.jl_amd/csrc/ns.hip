@@ -194,6 +194,52 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, 
   }
 }
 
+// The mask of one subband over the (k2, k3) plane of one k1: the group
+// table row k1 and each wedge plane as a base and (k2, k3) strides, so an
+// element costs three loads.  Same values, and the same product
+// G (w_a w_b), as ns_mask.
+struct PlaneMask {
+  const float *g, *wa, *wb;  // wa, wb null: factor 1
+  int a2, a3, b2, b3;
+  bool unit;
+};
+__device__ __forceinline__ const float *plane_wedge(const NSMaskView &v, int32_t off, int32_t pair, int k1, int &s2,
+                                                    int &s3) {
+  if (off < 0) return nullptr;
+  const float *base = v.planes + off;
+  if (pair == 0) {  // (k1, k2)
+    s2 = 1;
+    s3 = 0;
+    return base + (size_t)k1 * v.n2;
+  }
+  if (pair == 1) {  // (k1, k3)
+    s2 = 0;
+    s3 = 1;
+    return base + (size_t)k1 * v.n3;
+  }
+  s2 = v.n3;  // (k2, k3)
+  s3 = 1;
+  return base;
+}
+__device__ __forceinline__ PlaneMask plane_mask(const NSMaskView &v, const NSItem &it, int k1, bool analysis) {
+  PlaneMask m{};
+  m.g = ns_group_table(v, ns_group_index(v, it), analysis) + (size_t)k1 * v.n2 * v.n3;
+  m.unit = it.lowpass || ns_unit_wedge(v, analysis);
+  if (!m.unit) {
+    m.wa = plane_wedge(v, it.pa_off, it.pa_pair, k1, m.a2, m.a3);
+    m.wb = plane_wedge(v, it.pb_off, it.pb_pair, k1, m.b2, m.b3);
+  }
+  return m;
+}
+// Mask at plane element i = k2 n3 + k3.
+__device__ __forceinline__ float plane_mask_at(const PlaneMask &m, int i, int k2, int k3) {
+  const float g = m.g[i];
+  if (m.unit) return g * 1.0f;
+  const float wa = m.wa ? m.wa[k2 * m.a2 + k3 * m.a3] : 1.0f;
+  const float wb = m.wb ? m.wb[k2 * m.b2 + k3 * m.b3] : 1.0f;
+  return g * (wa * wb);
+}
+
 constexpr int AXIS_MAXE = 16;  // k_axis elements per thread: n G <= 4096
 
 // Complex FFT along axis AXIS (2 or 3) of a plane-major spectrum, G pencils
@@ -219,6 +265,7 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
   for (int k = tid; k < n; k += NT) tw[k] = twg[pl.tw + k];
   const NSItem it = items[blockIdx.y];
   // Pencil g of the block: first element at start + g gs, elements t es apart.
+  // An axis-2 block lies in one k1 plane: its mask is a PlaneMask there.
   int k1, q0 = 0, u0 = 0, npen;
   int64_t start, gs, es;
   if (AXIS == 3) {
@@ -237,6 +284,8 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
     gs = 1;
     es = n3;
   }
+  PlaneMask pm{};
+  if (AXIS == 2 && MODE != 0) pm = plane_mask(mv, it, k1, MODE == 1);
   // Axis 3 maps consecutive threads along the pencil, axis 2 across pencils:
   // both read and write runs of consecutive words.
   auto coords = [&](int idx, int &t, int &g) {
@@ -269,7 +318,7 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
       if (MODE == 1) {
         int b1, b2, b3;
         bin(t, g, b1, b2, b3);
-        val = cscale(val, ns_mask(mv, it, b1, b2, b3, true));
+        val = cscale(val, AXIS == 2 ? plane_mask_at(pm, b2 * n3 + b3, b2, b3) : ns_mask(mv, it, b1, b2, b3, true));
       }
     }
     a[t * G + g] = val;
@@ -284,7 +333,7 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
     if (MODE == 2) {
       int b1, b2, b3;
       bin(t, g, b1, b2, b3);
-      r = cscale(r, ns_mask(mv, it, b1, b2, b3, false));
+      r = cscale(r, AXIS == 2 ? plane_mask_at(pm, b2 * n3 + b3, b2, b3) : ns_mask(mv, it, b1, b2, b3, false));
     }
     out[start + g * gs + t * es] = r;
   }
@@ -373,52 +422,6 @@ __device__ void plane_fft(int poff, int twoff, const FftPlan1D &pl, int npen, in
   }
 }
 
-// The mask of one subband over the (k2, k3) plane of one k1: the group
-// table row k1 and each wedge plane as a base and (k2, k3) strides, so an
-// element costs three loads.  Same values, and the same product
-// G (w_a w_b), as ns_mask.
-struct PlaneMask {
-  const float *g, *wa, *wb;  // wa, wb null: factor 1
-  int a2, a3, b2, b3;
-  bool unit;
-};
-__device__ __forceinline__ const float *plane_wedge(const NSMaskView &v, int32_t off, int32_t pair, int k1, int &s2,
-                                                    int &s3) {
-  if (off < 0) return nullptr;
-  const float *base = v.planes + off;
-  if (pair == 0) {  // (k1, k2)
-    s2 = 1;
-    s3 = 0;
-    return base + (size_t)k1 * v.n2;
-  }
-  if (pair == 1) {  // (k1, k3)
-    s2 = 0;
-    s3 = 1;
-    return base + (size_t)k1 * v.n3;
-  }
-  s2 = v.n3;  // (k2, k3)
-  s3 = 1;
-  return base;
-}
-__device__ __forceinline__ PlaneMask plane_mask(const NSMaskView &v, const NSItem &it, int k1, bool analysis) {
-  PlaneMask m{};
-  m.g = ns_group_table(v, ns_group_index(v, it), analysis) + (size_t)k1 * v.n2 * v.n3;
-  m.unit = it.lowpass || ns_unit_wedge(v, analysis);
-  if (!m.unit) {
-    m.wa = plane_wedge(v, it.pa_off, it.pa_pair, k1, m.a2, m.a3);
-    m.wb = plane_wedge(v, it.pb_off, it.pb_pair, k1, m.b2, m.b3);
-  }
-  return m;
-}
-// Mask at plane element i = k2 n3 + k3.
-__device__ __forceinline__ float plane_mask_at(const PlaneMask &m, int i, int k2, int k3) {
-  const float g = m.g[i];
-  if (m.unit) return g * 1.0f;
-  const float wa = m.wa ? m.wa[k2 * m.a2 + k3 * m.a3] : 1.0f;
-  const float wb = m.wb ? m.wb[k2 * m.b2 + k3 * m.b3] : 1.0f;
-  return g * (wa * wb);
-}
-
 // 2-D FFT over the (k2, k3) plane of one k1 (blockIdx.x) of a plane-major
 // spectrum, the plane held in LDS with row pitch n3p (odd, so that strided
 // pencils fall in distinct banks).  Modes as k_axis: 0 plain, 1 mask before
@@ -458,6 +461,65 @@ __global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_plane(const float2 *in, int
     float2 v = P[k2 * n3p + k3];
     if (MODE == 2) v = cscale(v, plane_mask_at(pm, i, k2, k3));
     out[base + i] = v;
+  }
+}
+
+// FFT along axis AXIS (2 or 3) of a plane-major spectrum for a strip of
+// G pencils held in LDS, with the plane kernel's in-place passes: for axis 3,
+// G consecutive (k1, k2) rows (one contiguous run, LDS [g][t]); for axis 2,
+// G consecutive k3 of one k1 (runs of G words, LDS [t][g]).  The LDS pitch
+// is odd.  Modes as k_axis; an axis-2 strip lies in one k1 plane, so its
+// mask is a PlaneMask there (axis-3 strips are MODE 0 only).  For planes
+// too large for k_plane; k_axis stays for lengths the strip cannot take.
+template <int AXIS, int DIR, int MODE, bool TWO>
+__global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_strip(const float2 *in, int64_t in_bs, float2 *out,
+                                                           int64_t out_bs, int h1, int n2, int n3, int G, int pitch,
+                                                           FftPlan1D pl, const float2 *__restrict__ twg,
+                                                           NSMaskView mv, const NSItem *__restrict__ items) {
+  static_assert(AXIS == 2 || MODE == 0, "axis-3 strips are unmasked");
+  in += blockIdx.y * in_bs;
+  out += blockIdx.y * out_bs;
+  extern __shared__ float2 smem[];
+  const int n = AXIS == 2 ? n2 : n3;
+  float2 *S = smem + n;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < n; k += PT) smem[k] = twg[pl.tw + k];
+  int64_t start;
+  int npen, k1 = 0, u0 = 0;
+  if (AXIS == 3) {
+    const int q0 = blockIdx.x * G;
+    npen = min(G, h1 * n2 - q0);
+    start = (int64_t)q0 * n3;
+  } else {
+    const int ub = (n3 + G - 1) / G;
+    k1 = blockIdx.x / ub;
+    u0 = (blockIdx.x - k1 * ub) * G;
+    npen = min(G, n3 - u0);
+    start = (int64_t)k1 * n2 * n3 + u0;
+  }
+  PlaneMask pm{};
+  if (AXIS == 2 && MODE != 0) pm = plane_mask(mv, items[blockIdx.y], k1, MODE == 1);
+  // Element e of the strip: axis 3 e = g n3 + t at start + e; axis 2
+  // e = t npen + g at start + t n3 + g.
+  const int inner = AXIS == 3 ? n3 : npen;
+  const int nel = n * npen;
+  const FastDiv dd = fast_div(inner);
+  for (int e = tid; e < nel; e += PT) {
+    const int hi = div_of(e, dd), lo = e - hi * inner;
+    float2 v = in[AXIS == 3 ? start + e : start + (int64_t)hi * n3 + lo];
+    if (AXIS == 2 && MODE == 1) v = cscale(v, plane_mask_at(pm, hi * n3 + u0 + lo, hi, u0 + lo));
+    S[hi * pitch + lo] = v;
+  }
+  __syncthreads();
+  if (AXIS == 3)
+    plane_fft<DIR>(n, 0, pl, npen, 1, pitch, tid);
+  else
+    plane_fft<DIR>(n, 0, pl, npen, pitch, 1, tid);
+  for (int e = tid; e < nel; e += PT) {
+    const int hi = div_of(e, dd), lo = e - hi * inner;
+    float2 v = S[hi * pitch + lo];
+    if (AXIS == 2 && MODE == 2) v = cscale(v, plane_mask_at(pm, hi * n3 + u0 + lo, hi, u0 + lo));
+    out[AXIS == 3 ? start + e : start + (int64_t)hi * n3 + lo] = v;
   }
 }
 
@@ -517,6 +579,7 @@ struct surfhip_ns_plan {
   int g_rows = 16, g_axis = 16;
   int n3p = 1;        // plane row pitch: n3 rounded up to odd
   bool plane = false;  // axes 2 and 3 by the plane kernel
+  bool strip[2] = {false, false};  // axis 2 / axis 3 by the strip kernel (when not plane)
   int nwork = 1;                    // subbands per general-path launch (work holds nwork spectra)
   float tuned_ms = 0.0f;            // timed sequence of the chosen configuration (0: not tuned)
   DevBuf<float> gana, gsyn, planes;
@@ -829,8 +892,36 @@ struct surfhip_ns_plan {
     });
     SH_LAUNCHED();
   }
+  // Strip kernel of axis a (2 or 3): pencils per strip, LDS pitch and bytes.
+  int strip_g(int a) const { return std::max(1, std::min(32, 8192 / (a == 2 ? n2 : n3))); }
+  int strip_pitch(int a) const { return a == 2 ? (strip_g(2) | 1) : (n3 | 1); }
+  size_t lds_strip(int a) const {
+    const int n = a == 2 ? n2 : n3;
+    return sizeof(float2) * ((size_t)n + (size_t)(a == 2 ? n2 : strip_g(3)) * strip_pitch(a));
+  }
+  // Whether the strip kernel takes axis a: every radix hand-written and each
+  // pass's butterflies held in registers.
+  bool strip_fits(int a) const {
+    const FftPlan1D &pl = a == 2 ? f2 : f3;
+    if (lds_strip(a) > PLANE_LDS) return false;
+    for (int k = 0; k < pl.npass; ++k)
+      if (!plane_radix_fits(pl.radix[k], (int64_t)pl.n * strip_g(a))) return false;
+    return true;
+  }
   template <int AXIS, int DIR, int MODE>
   void axis(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
+    if (strip[AXIS - 2]) {
+      const int G = strip_g(AXIS);
+      const int grid = AXIS == 3 ? (h1 * n2 + G - 1) / G : h1 * ((n3 + G - 1) / G);
+      const size_t lds = lds_strip(AXIS);
+      auto *kernel = 2 * lds <= PLANE_LDS ? &k_strip<AXIS, DIR, MODE, true> : &k_strip<AXIS, DIR, MODE, false>;
+      timed(DIR < 0 ? 2 : 3, [&] {
+        hipLaunchKernelGGL(kernel, dim3(grid, nb), dim3(PT), lds, stream, in, in_bs, out, out_bs, h1, n2, n3, G,
+                           strip_pitch(AXIS), AXIS == 2 ? f2 : f3, tw.p, dview, ditems.p + item0);
+      });
+      SH_LAUNCHED();
+      return;
+    }
     const int G = g_axis;
     const int grid = AXIS == 3 ? (h1 * n2 + G - 1) / G : h1 * ((n3 + G - 1) / G);
     const FftPlan1D &pl = AXIS == 2 ? f2 : f3;
@@ -844,8 +935,7 @@ struct surfhip_ns_plan {
     SH_LAUNCHED();
   }
   // FFT over axes 2 and 3 (DIR), modes as k_axis: the plane kernel where the
-  // plane fits, else one k_axis pass per axis (the mask on the pass that
-  // touches the spectrum first (MODE 1) or last (MODE 2)).
+  // plane fits, else one k_axis pass per axis.
   template <int DIR, int MODE>
   void axes23(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
     if (plane) {
@@ -857,8 +947,15 @@ struct surfhip_ns_plan {
       SH_LAUNCHED();
       return;
     }
-    axis<3, DIR, MODE == 1 ? 1 : 0>(in, out, item0, nb, in_bs, out_bs);
-    axis<2, DIR, MODE == 2 ? 2 : 0>(out, out, item0, nb, out_bs, out_bs);
+    // Both masks on the axis-2 pass, whose blocks each lie in one k1 plane:
+    // analysis transforms axis 2 first, synthesis last.
+    if (MODE == 1) {
+      axis<2, DIR, 1>(in, out, item0, nb, in_bs, out_bs);
+      axis<3, DIR, 0>(out, out, item0, nb, out_bs, out_bs);
+    } else {
+      axis<3, DIR, 0>(in, out, item0, nb, in_bs, out_bs);
+      axis<2, DIR, MODE == 2 ? 2 : 0>(out, out, item0, nb, out_bs, out_bs);
+    }
   }
   // acc += work_0 + ... + work_{nb-1} in order.
   void sum_into(float2 *a, const float2 *w, int nb) {
@@ -1213,6 +1310,22 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       // selects the per-axis passes, for comparison).
       P->n3p = n3 | 1;
       P->plane = std::getenv("SURFHIP_NS_NO_PLANE") == nullptr && P->plane_fits();
+      if (!P->plane && std::getenv("SURFHIP_NS_NO_STRIP") == nullptr)
+        for (int a = 2; a <= 3; ++a) {
+          P->strip[a - 2] = P->strip_fits(a);
+          if (!P->strip[a - 2]) continue;
+          const int lds = (int)P->lds_strip(a);
+          const void *k2[] = {(const void *)&k_strip<2, -1, 0, true>,  (const void *)&k_strip<2, 1, 0, true>,
+                              (const void *)&k_strip<2, 1, 1, true>,   (const void *)&k_strip<2, -1, 2, true>,
+                              (const void *)&k_strip<2, -1, 0, false>, (const void *)&k_strip<2, 1, 0, false>,
+                              (const void *)&k_strip<2, 1, 1, false>,  (const void *)&k_strip<2, -1, 2, false>};
+          const void *k3[] = {(const void *)&k_strip<3, -1, 0, true>, (const void *)&k_strip<3, 1, 0, true>,
+                              (const void *)&k_strip<3, -1, 0, false>, (const void *)&k_strip<3, 1, 0, false>};
+          if (a == 2)
+            for (const void *k : k2) SH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+          else
+            for (const void *k : k3) SH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        }
       if (P->plane) {
         const int lds = (int)P->lds_plane();
         for (const void *k :

@@ -91,21 +91,27 @@ def test_ns_analyze_synthesize_match_oracle(dims, J, L, bo, tight):
     assert torch.equal(xr, y2)
 
 
+@pytest.mark.parametrize("route", ["strip", "axis"])
 @pytest.mark.parametrize("dims,J,L,bo,tight", [CASES[1], CASES[2], CASES[4], CASES[6], CASES[11]])
-def test_ns_per_axis_passes_match_oracle(dims, J, L, bo, tight, monkeypatch):
+def test_ns_per_axis_passes_match_oracle(dims, J, L, bo, tight, route, monkeypatch):
     """General sizes run axes 2 and 3 as one plane kernel where a (k2, k3)
-    plane fits one block's LDS, else as one FFT pass per axis.  The per-axis
-    route on the same cases: oracle parity, agreement with the plane route,
-    and streaming synthesis equal to the materialised one bit for bit."""
+    plane fits one block's LDS, else as one FFT pass per axis: a strip kernel
+    (in-place passes over up to 32 pencils) where its radices allow, else the
+    Stockham pencil kernel.  Both per-axis routes on the same cases: oracle
+    parity, agreement with the plane route, and streaming synthesis equal to
+    the materialised one bit for bit."""
     dev = _dev()
-    print(f"case dims={dims} J={J} L={L} bo={bo} tight={tight} per-axis", flush=True)
+    print(f"case dims={dims} J={J} L={L} bo={bo} tight={tight} route={route}", flush=True)
     monkeypatch.setenv("SURFHIP_NS_NO_PLANE", "1")
+    if route == "axis":
+        monkeypatch.setenv("SURFHIP_NS_NO_STRIP", "1")
     test_ns_analyze_synthesize_match_oracle(dims, J, L, bo, tight)
     rng = np.random.default_rng(17)
     x = from_numpy(rng.standard_normal(dims).astype(np.float32), dev)
     t = N.NSSurfacelet(scales=J, direction_levels=list(L), bernstein_order=bo, tight=tight)
     ws_axis = N.make_workspace(t, x)
     monkeypatch.delenv("SURFHIP_NS_NO_PLANE")
+    monkeypatch.delenv("SURFHIP_NS_NO_STRIP", raising=False)
     ws_plane = N.make_workspace(t, x)
     c_axis = N.analyze(t, x, workspace=ws_axis)
     c_plane = N.analyze(t, x, workspace=ws_plane)
