@@ -585,6 +585,7 @@ struct surfhip_ns_plan {
   DevBuf<float> gana, gsyn, planes;
   DevBuf<float2> tw, X[2], acc[2], work;
   DevBuf<NSItem> ditems;
+  DevBuf<NSItem> dvisit;  // items in visiting order: 1 .. S-1, then the lowpass 0
   NSMaskView dview{};
   bool streaming_fwd = false;
 
@@ -909,7 +910,9 @@ struct surfhip_ns_plan {
     return true;
   }
   template <int AXIS, int DIR, int MODE>
-  void axis(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
+  void axis(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0,
+            const NSItem *items = nullptr) {
+    if (!items) items = ditems.p + item0;
     if (strip[AXIS - 2]) {
       const int G = strip_g(AXIS);
       const int grid = AXIS == 3 ? (h1 * n2 + G - 1) / G : h1 * ((n3 + G - 1) / G);
@@ -917,7 +920,7 @@ struct surfhip_ns_plan {
       auto *kernel = 2 * lds <= PLANE_LDS ? &k_strip<AXIS, DIR, MODE, true> : &k_strip<AXIS, DIR, MODE, false>;
       timed(DIR < 0 ? 2 : 3, [&] {
         hipLaunchKernelGGL(kernel, dim3(grid, nb), dim3(PT), lds, stream, in, in_bs, out, out_bs, h1, n2, n3, G,
-                           strip_pitch(AXIS), AXIS == 2 ? f2 : f3, tw.p, dview, ditems.p + item0);
+                           strip_pitch(AXIS), AXIS == 2 ? f2 : f3, tw.p, dview, items);
       });
       SH_LAUNCHED();
       return;
@@ -930,19 +933,21 @@ struct surfhip_ns_plan {
     auto *kernel = fft_plan_generic(pl) ? &k_axis<AXIS, DIR, MODE, true> : &k_axis<AXIS, DIR, MODE, false>;
     timed(DIR < 0 ? 2 : 3, [&] {
       hipLaunchKernelGGL(kernel, dim3(grid, nb), dim3(NT), lds_axis(pl.n), stream, in, in_bs, out, out_bs, h1, n2,
-                         n3, G, pl, tw.p, dview, ditems.p + item0);
+                         n3, G, pl, tw.p, dview, items);
     });
     SH_LAUNCHED();
   }
   // FFT over axes 2 and 3 (DIR), modes as k_axis: the plane kernel where the
   // plane fits, else one k_axis pass per axis.
   template <int DIR, int MODE>
-  void axes23(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0) {
+  void axes23(const float2 *in, float2 *out, int item0 = 0, int nb = 1, int64_t in_bs = 0, int64_t out_bs = 0,
+              const NSItem *items = nullptr) {
+    if (!items) items = ditems.p + item0;
     if (plane) {
       auto *kernel = plane_two() ? &k_plane<DIR, MODE, true> : &k_plane<DIR, MODE, false>;
       timed(DIR < 0 ? 2 : 3, [&] {
         hipLaunchKernelGGL(kernel, dim3(h1, nb), dim3(PT), lds_plane(), stream, in, in_bs, out, out_bs, n2, n3, n3p,
-                           f2, f3, tw.p, dview, ditems.p + item0);
+                           f2, f3, tw.p, dview, items);
       });
       SH_LAUNCHED();
       return;
@@ -950,11 +955,11 @@ struct surfhip_ns_plan {
     // Both masks on the axis-2 pass, whose blocks each lie in one k1 plane:
     // analysis transforms axis 2 first, synthesis last.
     if (MODE == 1) {
-      axis<2, DIR, 1>(in, out, item0, nb, in_bs, out_bs);
-      axis<3, DIR, 0>(out, out, item0, nb, out_bs, out_bs);
+      axis<2, DIR, 1>(in, out, item0, nb, in_bs, out_bs, items);
+      axis<3, DIR, 0>(out, out, item0, nb, out_bs, out_bs, items);
     } else {
-      axis<3, DIR, 0>(in, out, item0, nb, in_bs, out_bs);
-      axis<2, DIR, MODE == 2 ? 2 : 0>(out, out, item0, nb, out_bs, out_bs);
+      axis<3, DIR, 0>(in, out, item0, nb, in_bs, out_bs, items);
+      axis<2, DIR, MODE == 2 ? 2 : 0>(out, out, item0, nb, out_bs, out_bs, items);
     }
   }
   // acc += work_0 + ... + work_{nb-1} in order.
@@ -1102,6 +1107,17 @@ struct surfhip_ns_plan {
   void analyze(const float *x, float *coeffs) {
     forward_spectrum(x);
     if (!fused) {
+      if (nwork >= T.S) {
+        // Every subband in one batch, in visiting order (work slot b holds
+        // subband b + 1, the last slot the lowpass).
+        const float scale = 1.0f / (float)N;
+        for (int ch = 0; ch < nch; ++ch) {
+          axes23<1, 1>(X[ch].p, work.p, 0, T.S, 0, H, dvisit.p);
+          c2r(work.p, coeffs + (int64_t)N * nch + ch, nch, scale, T.S - 1, H, N * nch);
+          c2r(work.p + (int64_t)(T.S - 1) * H, coeffs + ch, nch, scale);
+        }
+        return;
+      }
       subbands_generic(1, T.S - 1, coeffs + (int64_t)N * nch);
       subbands_generic(0, 1, coeffs);
       return;
@@ -1168,8 +1184,19 @@ struct surfhip_ns_plan {
   void synthesize(const float *coeffs, float *x) {
     if (!fused) {
       inverse_begin();
-      accumulate_generic(1, T.S - 1, coeffs + (int64_t)N * nch);
-      accumulate_generic(0, 1, coeffs);
+      if (nwork >= T.S) {
+        // Every subband in one batch, summed in visiting order: the order of
+        // accumulate_generic over 1 .. S-1 and then the lowpass.
+        for (int ch = 0; ch < nch; ++ch) {
+          r2c(coeffs + (int64_t)N * nch + ch, nch, work.p, T.S - 1, N * nch, H);
+          r2c(coeffs + ch, nch, work.p + (int64_t)(T.S - 1) * H);
+          axes23<-1, 2>(work.p, work.p, 0, T.S, H, H, dvisit.p);
+          sum_into(acc[ch].p, work.p, T.S);
+        }
+      } else {
+        accumulate_generic(1, T.S - 1, coeffs + (int64_t)N * nch);
+        accumulate_generic(0, 1, coeffs);
+      }
       finish(x);
       return;
     }
@@ -1365,6 +1392,11 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
     if (P->T.planes.empty()) P->T.planes.push_back(0.0f);
     P->planes.upload(P->T.planes);
     P->ditems.upload(P->T.items);
+    {
+      std::vector<NSItem> visit(P->T.items.begin() + 1, P->T.items.end());
+      visit.push_back(P->T.items[0]);
+      P->dvisit.upload(visit);
+    }
     P->dview = NSMaskView{P->gana.p, P->T.tight ? P->gana.p : P->gsyn.p, P->planes.p, P->h1, n2, n3,
                           P->T.ngroups, P->T.tight, 0, 0};
     for (int ch = 0; ch < P->nch; ++ch) P->X[ch].alloc(P->H);
@@ -1377,7 +1409,7 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       // work spectra within 1 GiB (at least 8 of them).  Large batches make
       // launches of many blocks, so the last wave of blocks costs little.
       const int64_t fit = ((int64_t)1 << 30) / ((int64_t)sizeof(float2) * P->H);
-      P->nwork = (int)std::max<int64_t>(1, std::min<int64_t>(P->T.S - 1, std::max<int64_t>(8, fit)));
+      P->nwork = (int)std::max<int64_t>(1, std::min<int64_t>(P->T.S, std::max<int64_t>(8, fit)));
       P->work.alloc((size_t)P->H * P->nwork);
     }
     // The group-table inputs are no longer needed on the host.
