@@ -50,10 +50,13 @@ template <bool GEN>
 __global__ __launch_bounds__(NT) void k_rows_r2c(const float *__restrict__ x, int64_t xs, int64_t x_bs,
                                                  float2 *__restrict__ out, int64_t out_bs, int n1, int h1,
                                                  int n2, int n3, int G, FftPlan1D pl,
-                                                 const float2 *__restrict__ twg, int vec) {
-  // Batch entry blockIdx.y: its own input and output.
-  x += blockIdx.y * x_bs;
-  out += blockIdx.y * out_bs;
+                                                 const float2 *__restrict__ twg, int vec, int wrap_at,
+                                                 int64_t wrap_off) {
+  // Batch entry blockIdx.y: its own input and output; entries from wrap_at on
+  // read x + wrap_off + (b - wrap_at) x_bs instead.
+  const int eb = blockIdx.y;
+  x += eb < wrap_at ? eb * x_bs : wrap_off + (eb - wrap_at) * x_bs;
+  out += eb * out_bs;
   extern __shared__ float2 smem[];
   const int nrows = n2 * n3;
   const int P = G / 2;
@@ -118,9 +121,12 @@ template <bool GEN>
 __global__ __launch_bounds__(NT) void k_rows_c2r(const float2 *__restrict__ in, int64_t in_bs,
                                                  float *__restrict__ out, int64_t os, int64_t out_bs, float scale,
                                                  int n1, int h1, int n2, int n3, int G, FftPlan1D pl,
-                                                 const float2 *__restrict__ twg, int vec) {
-  in += blockIdx.y * in_bs;
-  out += blockIdx.y * out_bs;
+                                                 const float2 *__restrict__ twg, int vec, int wrap_at,
+                                                 int64_t wrap_off) {
+  // Entries from wrap_at on write out + wrap_off + (b - wrap_at) out_bs.
+  const int eb = blockIdx.y;
+  in += eb * in_bs;
+  out += eb < wrap_at ? eb * out_bs : wrap_off + (eb - wrap_at) * out_bs;
   extern __shared__ float2 smem[];
   const int nrows = n2 * n3;
   const int P = G / 2;
@@ -872,24 +878,32 @@ struct surfhip_ns_plan {
 
   // General-path launchers; nb > 1 runs a batch of nb subbands (grid.y),
   // entry b at in + b in_bs / out + b out_bs with item item0 + b.
-  void r2c(const float *x, int64_t xs, float2 *out, int nb = 1, int64_t x_bs = 0, int64_t out_bs = 0) {
+  // Entries from wrap_at on (if any) take their input at x + wrap_off.
+  void r2c(const float *x, int64_t xs, float2 *out, int nb = 1, int64_t x_bs = 0, int64_t out_bs = 0,
+           int wrap_at = -1, int64_t wrap_off = 0) {
+    if (wrap_at < 0) wrap_at = nb;
     int grid = (rows() + g_rows - 1) / g_rows;
-    const int vec = xs == 1 && n1 % 4 == 0 && x_bs % 4 == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    const int vec = xs == 1 && n1 % 4 == 0 && x_bs % 4 == 0 && wrap_off % 4 == 0 &&
+                    (reinterpret_cast<uintptr_t>(x) & 15) == 0;
     timed(0, [&] {
       hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_r2c<true> : k_rows_r2c<false>, dim3(grid, nb), dim3(NT),
-                         lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1, h1, n2, n3, g_rows, f1, tw.p, vec);
+                         lds_rows_r2c(), stream, x, xs, x_bs, out, out_bs, n1, h1, n2, n3, g_rows, f1, tw.p, vec,
+                         wrap_at, wrap_off);
     });
     SH_LAUNCHED();
   }
+  // Entries from wrap_at on (if any) write at out + wrap_off.
   void c2r(const float2 *in, float *out, int64_t os, float scale, int nb = 1, int64_t in_bs = 0,
-           int64_t out_bs = 0) {
+           int64_t out_bs = 0, int wrap_at = -1, int64_t wrap_off = 0) {
+    if (wrap_at < 0) wrap_at = nb;
     int grid = (rows() + g_rows - 1) / g_rows;
     const int vin = rows() % 2 == 0 && in_bs % 2 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
-    const int vout = os == 1 && n1 % 4 == 0 && out_bs % 4 == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+    const int vout = os == 1 && n1 % 4 == 0 && out_bs % 4 == 0 && wrap_off % 4 == 0 &&
+                     (reinterpret_cast<uintptr_t>(out) & 15) == 0;
     timed(1, [&] {
       hipLaunchKernelGGL(fft_plan_generic(f1) ? k_rows_c2r<true> : k_rows_c2r<false>, dim3(grid, nb), dim3(NT),
                          lds_rows_c2r(), stream, in, in_bs, out, os, out_bs, scale, n1, h1, n2, n3, g_rows, f1, tw.p,
-                         vin | (vout << 1));
+                         vin | (vout << 1), wrap_at, wrap_off);
     });
     SH_LAUNCHED();
   }
@@ -1113,8 +1127,8 @@ struct surfhip_ns_plan {
         const float scale = 1.0f / (float)N;
         for (int ch = 0; ch < nch; ++ch) {
           axes23<1, 1>(X[ch].p, work.p, 0, T.S, 0, H, dvisit.p);
-          c2r(work.p, coeffs + (int64_t)N * nch + ch, nch, scale, T.S - 1, H, N * nch);
-          c2r(work.p + (int64_t)(T.S - 1) * H, coeffs + ch, nch, scale);
+          // slot S-1 (the lowpass) writes subband 0, N nch before subband 1
+          c2r(work.p, coeffs + (int64_t)N * nch + ch, nch, scale, T.S, H, N * nch, T.S - 1, -(int64_t)N * nch);
         }
         return;
       }
@@ -1188,8 +1202,7 @@ struct surfhip_ns_plan {
         // Every subband in one batch, summed in visiting order: the order of
         // accumulate_generic over 1 .. S-1 and then the lowpass.
         for (int ch = 0; ch < nch; ++ch) {
-          r2c(coeffs + (int64_t)N * nch + ch, nch, work.p, T.S - 1, N * nch, H);
-          r2c(coeffs + ch, nch, work.p + (int64_t)(T.S - 1) * H);
+          r2c(coeffs + (int64_t)N * nch + ch, nch, work.p, T.S, N * nch, H, T.S - 1, -(int64_t)N * nch);
           axes23<-1, 2>(work.p, work.p, 0, T.S, H, H, dvisit.p);
           sum_into(acc[ch].p, work.p, T.S);
         }
