@@ -58,6 +58,8 @@ CASES = [
     ((128, 64, 32), 2, (2, 3), 12, True),
     ((32, 128, 64), 3, (2, 2, 3), 12, False),
     ((64, 64, 256), 2, (1, 2), 6, True),
+    # composite radices 14, 9, 6 and 12 (28 = 14 x 2, 54 = 9 x 6, 36 = 12 x 3)
+    ((28, 54, 36), 2, (1, 2), 6, True),
 ]
 
 
@@ -92,7 +94,7 @@ def test_ns_analyze_synthesize_match_oracle(dims, J, L, bo, tight):
 
 
 @pytest.mark.parametrize("route", ["strip", "axis"])
-@pytest.mark.parametrize("dims,J,L,bo,tight", [CASES[1], CASES[2], CASES[4], CASES[6], CASES[11]])
+@pytest.mark.parametrize("dims,J,L,bo,tight", [CASES[1], CASES[2], CASES[4], CASES[6], CASES[11], CASES[12]])
 def test_ns_per_axis_passes_match_oracle(dims, J, L, bo, tight, route, monkeypatch):
     """General sizes run axes 2 and 3 as one plane kernel where a (k2, k3)
     plane fits one block's LDS, else as one FFT pass per axis: a strip kernel
