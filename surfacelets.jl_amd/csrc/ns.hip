@@ -1340,7 +1340,9 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
     P->f3 = make_fft_plan(n3, pool);
     P->fused = std::getenv("SURFHIP_NS_GENERIC") == nullptr && fused_ops_for(n1, n2, n3, P->fops);
     if (!P->fused) {
-      P->g_rows = std::max(2, std::min(16, pick_group(n1)));
+      // Rows per axis-1 block: the most (up to 32) whose LDS fits 64 KiB.
+      P->g_rows = 32;
+      while (P->g_rows > 2 && std::max(P->lds_rows_r2c(), P->lds_rows_c2r()) > 65536) P->g_rows /= 2;
       P->g_axis = std::min(pick_group(std::max(n2, n3)), 16);
       check_lds(P->lds_rows_r2c(), "the axis-1 transform");
       check_lds(P->lds_rows_c2r(), "the axis-1 transform");
