@@ -44,7 +44,7 @@ __device__ __forceinline__ int64_t lin3(int i1, int i2, int i3, int m1, int m2) 
 // segments of length `seg` (seg divides the axis; seg = the axis length for a
 // whole transform).  G pencils per block: consecutive segments for AXIS 1
 // (one contiguous run), consecutive i1 for AXIS 2 and 3.
-template <int AXIS, int DIR>
+template <int AXIS, int DIR, bool GEN>
 __global__ __launch_bounds__(DT) void k_c2c(const float2 *in, float2 *out, int m1, int m2, int m3, int seg, int G,
                                            FftPlan1D pl, const float2 *__restrict__ twg) {
   extern __shared__ float2 smem[];
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(DT) void k_c2c(const float2 *in, float2 *out, int m
     a[t * G + g] = g < valid ? in[base0 + g * sg + t * sa] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
-  float2 *res = lds_fft<DIR>(a, b, pl, G, tw, tid, DT);
+  float2 *res = lds_fft<DIR, GEN>(a, b, pl, G, tw, tid, DT);
   for (int idx = tid; idx < seg * G; idx += DT) {
     int t, g;
     if (AXIS == 1) {
@@ -524,6 +524,7 @@ __device__ __forceinline__ int64_t block_stride(const BlockMap &M) {
 // coefficients (element stride os).  G lines per block, two per complex FFT:
 // lines 2f and 2f + 1 have Hermitian spectra A and B, so the inverse of
 // A + iB is line 2f + i line 2f + 1.
+template <bool GEN>
 __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, float *__restrict__ out, int64_t os,
                                                  BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
   extern __shared__ float2 smem[];
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, 
     a[idx] = make_float2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  float2 *res = lds_fft<1>(a, b, pl, F, tw, tid, DT);
+  float2 *res = lds_fft<1, GEN>(a, b, pl, F, tw, tid, DT);
   for (int idx = tid; idx < nK * G; idx += DT) {
     int t, g;
     if (M.K == 0) {
@@ -593,6 +594,7 @@ __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, 
 // unnormalised c2r along K.  Lines 2f and 2f + 1 share one complex FFT
 // Z = FFT(line 2f + i line 2f + 1): A[t] = (Z[t] + conj Z[-t]) / 2 and
 // B[t] = (Z[t] - conj Z[-t]) / 2i.
+template <bool GEN>
 __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, int64_t is, float2 *__restrict__ S,
                                                  BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg,
                                                  int dbl) {
@@ -624,7 +626,7 @@ __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, 
     af[2 * (t * F + g / 2) + (g & 1)] = g < valid ? in[(LB.c[g] + t * sC) * is] : 0.0f;
   }
   __syncthreads();
-  float2 *res = lds_fft<-1>(a, b, pl, F, tw, tid, DT);
+  float2 *res = lds_fft<-1, GEN>(a, b, pl, F, tw, tid, DT);
   for (int idx = tid; idx < hK * G; idx += DT) {
     int t, g;
     if (M.K == 0) {
@@ -1035,8 +1037,9 @@ struct surfhip_s_plan {
       const int64_t ub = (m[0] + G - 1) / G;
       blocks = axis == 1 ? ub * (m[1] / seg) * m[2] : ub * m[1] * (m[2] / seg);
     }
-#define SH_C2C(AX, D) \
-  go((k_c2c<AX, D>), dim3((unsigned)blocks), dim3(DT), lds, A, A, m[0], m[1], m[2], seg, G, pl, tw.p)
+#define SH_C2C(AX, D)                                                                                       \
+  go(fft_plan_generic(pl) ? &k_c2c<AX, D, true> : &k_c2c<AX, D, false>, dim3((unsigned)blocks), dim3(DT), lds, A, \
+     A, m[0], m[1], m[2], seg, G, pl, tw.p)
     if (axis == 0) { if (dir < 0) SH_C2C(1, -1); else SH_C2C(1, 1); }
     else if (axis == 1) { if (dir < 0) SH_C2C(2, -1); else SH_C2C(2, 1); }
     else { if (dir < 0) SH_C2C(3, -1); else SH_C2C(3, 1); }
@@ -1208,11 +1211,13 @@ struct surfhip_s_plan {
     if (lds > 65536) fail(SURFHIP_ERR_ARG, fmt("axis length %d is too long for the decimated path", nK));
     const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
     const unsigned blocks = (unsigned)((nlines + G - 1) / G);
+    // Without the direct-DFT pass where every radix is hand-written.
+    const bool gen = fft_plan_generic(pl);
     if (c2r)
-      go(k_c2r_lines, dim3(blocks), dim3(DT), lds, S, coef, stride, M, G, pl, tw.p);
+      go(gen ? k_c2r_lines<true> : k_c2r_lines<false>, dim3(blocks), dim3(DT), lds, S, coef, stride, M, G, pl, tw.p);
     else
-      go(k_r2c_lines, dim3(blocks), dim3(DT), lds, (const float *)coef, stride, S, M, G, pl,
-                         tw.p, dbl);
+      go(gen ? k_r2c_lines<true> : k_r2c_lines<false>, dim3(blocks), dim3(DT), lds, (const float *)coef, stride, S, M,
+         G, pl, tw.p, dbl);
   }
 
   void analyze_channel(const float *x, float *coef) {
