@@ -2,16 +2,18 @@
 //
 // Every subband is irfft(X . M_k) with X = rfft(x) and M_k the real mask of
 // subband k (src/transforms/ns_surfacelet.jl:8-32); synthesis is
-// irfft(sum_k rfft(c_k) . M_k).  The 3-D real FFTs run as three LDS passes:
-// a paired-row r2c/c2r along axis 1 (the contiguous axis) and complex passes
-// along axes 2 and 3, each block holding G pencils that are contiguous along
-// k1 so that every global access is coalesced.  The mask is evaluated on the
-// fly from FP32 tables (built in FP64 on the host) inside the first inverse
-// pass of analysis and the last forward pass of synthesis, so no masked
-// spectrum is ever stored.
+// irfft(sum_k rfft(c_k) . M_k).  This file holds the general-size path and
+// the plan; the power-of-two grids run the factored kernels of ns_fused.hip.
+// A 3-D real FFT here is a paired-row r2c/c2r along axis 1 (the contiguous
+// axis of the volume) and one LDS-resident 2-D transform of each (k2, k3)
+// plane (k_plane), or where a plane exceeds LDS one pass per axis over
+// strips of pencils (k_strip, else k_axis).  The mask is evaluated on the
+// fly from FP32 tables (built in FP64) inside the plane or axis-2 pass:
+// before the inverse transform in analysis, after the forward one in
+// synthesis, whose masked spectra are then summed in subband order (k_sum).
 //
-// Spectra use the rfft grid in column-major order: S[k1 + h1 (k2 + n2 k3)],
-// h1 = n1/2 + 1.
+// Spectra use the rfft grid in plane-major order: S[(k1 n2 + k2) n3 + k3],
+// h1 = n1/2 + 1, the order of the group-factor tables.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -567,8 +569,8 @@ __global__ void k_zero(float2 *__restrict__ p, int64_t n) {
 using namespace surfhip;
 
 // A plan runs one of two kernel sets on the same tables: the factored path
-// (ns_fused.hip) on the power-of-two grids it covers, or the general
-// three-pass path above on any other size.  Both evaluate the same masks.
+// (ns_fused.hip) on the power-of-two grids it covers, or the general path
+// above on any other size.  Both evaluate the same masks.
 struct surfhip_ns_plan {
   NSTables T;
   int dtype = SURFHIP_F32;
