@@ -358,7 +358,9 @@ __global__ __launch_bounds__(NT) void k_axis(const float2 *in, int64_t in_bs, fl
 constexpr int PT = 1024;                 // threads of a plane block
 constexpr int PLANE_V = 16;              // values a thread holds across one pass
 constexpr size_t PLANE_LDS = 160 * 1024;  // LDS of one CU on gfx950
-template <int R, int DIR>
+// TWO: instantiated per register budget (a callee compiles to the tightest
+// budget among its callers).
+template <int R, int DIR, bool TWO>
 __device__ __noinline__ void plane_pass(int poff, int twoff, int n, int npen, int es, int ps, int Ns, int tid) {
   extern __shared__ float2 smem[];
   float2 *P = smem + poff;
@@ -407,24 +409,24 @@ inline bool plane_radix_fits(int R, int64_t elems) {
   return fft_radix_fixed(R) && (elems / R + PT - 1) / PT <= PLANE_V / R;
 }
 
-template <int DIR>
+template <int DIR, bool TWO>
 __device__ void plane_fft(int poff, int twoff, const FftPlan1D &pl, int npen, int es, int ps, int tid) {
   int Ns = 1;
   for (int p = 0; p < pl.npass; ++p) {
     const int R = pl.radix[p];
     switch (R) {
-      case 8: plane_pass<8, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 4: plane_pass<4, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 2: plane_pass<2, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 3: plane_pass<3, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 5: plane_pass<5, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 7: plane_pass<7, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 16: plane_pass<16, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 14: plane_pass<14, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 12: plane_pass<12, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 10: plane_pass<10, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      case 9: plane_pass<9, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
-      default: plane_pass<6, DIR>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 8: plane_pass<8, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 4: plane_pass<4, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 2: plane_pass<2, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 3: plane_pass<3, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 5: plane_pass<5, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 7: plane_pass<7, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 16: plane_pass<16, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 14: plane_pass<14, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 12: plane_pass<12, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 10: plane_pass<10, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      case 9: plane_pass<9, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
+      default: plane_pass<6, DIR, TWO>(poff, twoff, pl.n, npen, es, ps, Ns, tid); break;
     }
     Ns *= R;
   }
@@ -462,8 +464,8 @@ __global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_plane(const float2 *in, int
     P[k2 * n3p + k3] = v;
   }
   __syncthreads();
-  plane_fft<DIR>(n2 + n3, n2, p3, n2, 1, n3p, tid);  // along k3: pencils are rows
-  plane_fft<DIR>(n2 + n3, 0, p2, n3, n3p, 1, tid);   // along k2: pencils are columns
+  plane_fft<DIR, TWO>(n2 + n3, n2, p3, n2, 1, n3p, tid);  // along k3: pencils are rows
+  plane_fft<DIR, TWO>(n2 + n3, 0, p2, n3, n3p, 1, tid);   // along k2: pencils are columns
   for (int i = tid; i < nel; i += PT) {
     const int k2 = div_of(i, d3), k3 = i - k2 * n3;
     float2 v = P[k2 * n3p + k3];
@@ -520,9 +522,9 @@ __global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_strip(const float2 *in, int
   }
   __syncthreads();
   if (AXIS == 3)
-    plane_fft<DIR>(n, 0, pl, npen, 1, pitch, tid);
+    plane_fft<DIR, TWO>(n, 0, pl, npen, 1, pitch, tid);
   else
-    plane_fft<DIR>(n, 0, pl, npen, pitch, 1, tid);
+    plane_fft<DIR, TWO>(n, 0, pl, npen, pitch, 1, tid);
   for (int e = tid; e < nel; e += PT) {
     const int hi = div_of(e, dd), lo = e - hi * inner;
     float2 v = S[hi * pitch + lo];
