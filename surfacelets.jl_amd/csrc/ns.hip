@@ -1045,7 +1045,7 @@ struct surfhip_ns_plan {
     if (!plane)
       try_knob(&g_axis, {1, 2, 4, 8, 16, 32},
                [&] { return lds_axis(n2) <= 65536 && lds_axis(n3) <= 65536; });
-    try_knob(&g_rows, {2, 4, 8, 16, 32},
+    try_knob(&g_rows, {2, 4, 6, 8, 14, 16, 30, 32},
              [&] { return lds_rows_r2c() <= 65536 && lds_rows_c2r() <= 65536; });
     for (FftPlan1D *pl : {&f1, &f2, &f3}) {
       if (pl->npass < 2) continue;
@@ -1344,9 +1344,13 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
     P->f3 = make_fft_plan(n3, pool);
     P->fused = std::getenv("SURFHIP_NS_GENERIC") == nullptr && fused_ops_for(n1, n2, n3, P->fops);
     if (!P->fused) {
-      // Rows per axis-1 block: the most (up to 32) whose LDS fits 64 KiB.
-      P->g_rows = 32;
-      while (P->g_rows > 2 && std::max(P->lds_rows_r2c(), P->lds_rows_c2r()) > 65536) P->g_rows /= 2;
+      // Rows per axis-1 block: the most (up to 30) whose LDS fits 64 KiB, with
+      // an odd number of row pairs, so that the 16-byte row loads' and
+      // stores' transposing LDS accesses spread over 8 banks instead of one.
+      for (int g : {30, 14, 6, 2}) {
+        P->g_rows = g;
+        if (std::max(P->lds_rows_r2c(), P->lds_rows_c2r()) <= 65536) break;
+      }
       P->g_axis = std::min(pick_group(std::max(n2, n3)), 16);
       check_lds(P->lds_rows_r2c(), "the axis-1 transform");
       check_lds(P->lds_rows_c2r(), "the axis-1 transform");
