@@ -1125,19 +1125,17 @@ struct surfhip_ns_plan {
   void analyze(const float *x, float *coeffs) {
     forward_spectrum(x);
     if (!fused) {
-      if (nwork >= T.S) {
-        // Every subband in one batch, in visiting order (work slot b holds
-        // subband b + 1, the last slot the lowpass).
-        const float scale = 1.0f / (float)N;
-        for (int ch = 0; ch < nch; ++ch) {
-          axes23<1, 1>(X[ch].p, work.p, 0, T.S, 0, H, dvisit.p);
-          // slot S-1 (the lowpass) writes subband 0, N nch before subband 1
-          c2r(work.p, coeffs + (int64_t)N * nch + ch, nch, scale, T.S, H, N * nch, T.S - 1, -(int64_t)N * nch);
+      // Batches of nwork subbands in visiting order (position p holds subband
+      // p + 1, the last position the lowpass, subband 0, which sits N nch
+      // before subband 1: the rows kernels' wrap offset).
+      const float scale = 1.0f / (float)N;
+      for (int ch = 0; ch < nch; ++ch)
+        for (int p0 = 0; p0 < T.S; p0 += nwork) {
+          const int nb = std::min(nwork, T.S - p0);
+          axes23<1, 1>(X[ch].p, work.p, 0, nb, 0, H, dvisit.p + p0);
+          c2r(work.p, coeffs + (int64_t)(p0 + 1) * N * nch + ch, nch, scale, nb, H, N * nch, T.S - 1 - p0,
+              -(int64_t)(p0 + 1) * N * nch);
         }
-        return;
-      }
-      subbands_generic(1, T.S - 1, coeffs + (int64_t)N * nch);
-      subbands_generic(0, 1, coeffs);
       return;
     }
     const FusedCtx c = ctx();
@@ -1202,18 +1200,16 @@ struct surfhip_ns_plan {
   void synthesize(const float *coeffs, float *x) {
     if (!fused) {
       inverse_begin();
-      if (nwork >= T.S) {
-        // Every subband in one batch, summed in visiting order: the order of
-        // accumulate_generic over 1 .. S-1 and then the lowpass.
-        for (int ch = 0; ch < nch; ++ch) {
-          r2c(coeffs + (int64_t)N * nch + ch, nch, work.p, T.S, N * nch, H, T.S - 1, -(int64_t)N * nch);
-          axes23<-1, 2>(work.p, work.p, 0, T.S, H, H, dvisit.p);
-          sum_into(acc[ch].p, work.p, T.S);
+      // Batches in visiting order, summed in that order: the order of
+      // streaming accumulation over subbands 1 .. S-1 and then the lowpass.
+      for (int ch = 0; ch < nch; ++ch)
+        for (int p0 = 0; p0 < T.S; p0 += nwork) {
+          const int nb = std::min(nwork, T.S - p0);
+          r2c(coeffs + (int64_t)(p0 + 1) * N * nch + ch, nch, work.p, nb, N * nch, H, T.S - 1 - p0,
+              -(int64_t)(p0 + 1) * N * nch);
+          axes23<-1, 2>(work.p, work.p, 0, nb, H, H, dvisit.p + p0);
+          sum_into(acc[ch].p, work.p, nb);
         }
-      } else {
-        accumulate_generic(1, T.S - 1, coeffs + (int64_t)N * nch);
-        accumulate_generic(0, 1, coeffs);
-      }
       finish(x);
       return;
     }
