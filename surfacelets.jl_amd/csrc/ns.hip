@@ -534,11 +534,12 @@ __global__ __launch_bounds__(PT, TWO ? 8 : 4) void k_strip(const float2 *in, int
 }
 
 // acc += W_0 + W_1 + ... + W_{nb-1}, added in that order (W_b at W + b bs),
-// so a batch sums exactly as nb single-entry calls do.
+// so a batch sums exactly as nb single-entry calls do.  from_zero: the sum
+// starts from 0 instead of acc, as after zeroing acc.
 __global__ __launch_bounds__(NT) void k_sum(float2 *__restrict__ acc, const float2 *__restrict__ W, int64_t bs,
-                                            int nb, int64_t n) {
+                                            int nb, int64_t n, int from_zero) {
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-    float2 s = acc[i];
+    float2 s = from_zero ? make_float2(0.0f, 0.0f) : acc[i];
     for (int b = 0; b < nb; ++b) {
       const float2 w = W[b * bs + i];
       s = make_float2(s.x + w.x, s.y + w.y);
@@ -980,10 +981,12 @@ struct surfhip_ns_plan {
       axis<2, DIR, MODE == 2 ? 2 : 0>(out, out, item0, nb, out_bs, out_bs, items);
     }
   }
-  // acc += work_0 + ... + work_{nb-1} in order.
-  void sum_into(float2 *a, const float2 *w, int nb) {
+  // acc += work_0 + ... + work_{nb-1} in order (from_zero: acc = 0 first).
+  void sum_into(float2 *a, const float2 *w, int nb, int from_zero = 0) {
     const int64_t blocks = std::min<int64_t>((H + NT - 1) / NT, 4096);
-    timed(4, [&] { hipLaunchKernelGGL(k_sum, dim3((unsigned)blocks), dim3(NT), 0, stream, a, w, H, nb, H); });
+    timed(4, [&] {
+      hipLaunchKernelGGL(k_sum, dim3((unsigned)blocks), dim3(NT), 0, stream, a, w, H, nb, H, from_zero);
+    });
     SH_LAUNCHED();
   }
   void zero(float2 *p, int64_t n) {
@@ -1199,16 +1202,16 @@ struct surfhip_ns_plan {
   }
   void synthesize(const float *coeffs, float *x) {
     if (!fused) {
-      inverse_begin();
       // Batches in visiting order, summed in that order: the order of
       // streaming accumulation over subbands 1 .. S-1 and then the lowpass.
+      // The first batch's sum starts from zero, as after inverse_begin.
       for (int ch = 0; ch < nch; ++ch)
         for (int p0 = 0; p0 < T.S; p0 += nwork) {
           const int nb = std::min(nwork, T.S - p0);
           r2c(coeffs + (int64_t)(p0 + 1) * N * nch + ch, nch, work.p, nb, N * nch, H, T.S - 1 - p0,
               -(int64_t)(p0 + 1) * N * nch);
           axes23<-1, 2>(work.p, work.p, 0, nb, H, H, dvisit.p + p0);
-          sum_into(acc[ch].p, work.p, nb);
+          sum_into(acc[ch].p, work.p, nb, p0 == 0);
         }
       finish(x);
       return;
