@@ -329,7 +329,9 @@ def signal_size(t, c):
 
 
 def _check_coefficients(c, ws):
-    """src/transforms/surfacelet.jl:421-427."""
+    """src/transforms/surfacelet.jl:421-427, plus what the device needs: the
+    vector lives on the GPU (a host pointer must never reach a kernel)."""
+    check_device(c.data, "coefficient vector")
     if c.layout != ws.layout:
         raise DimensionMismatch(f"the coefficients' subband layout is not that of {ws.transform} on size {_jl_dims(ws.dims)}")
     if c.dtype != ws.dtype:
@@ -345,7 +347,14 @@ def analyze_(c, t, x, *, workspace=None, threaded=None):
     _check_coefficients(c, ws)
     xin = as_colmajor(x if x.dtype == ws.dtype else x.to(ws.dtype))
     ws.bind_stream()
-    _lib.call("surfhip_s_analyze", ws._plan, ptr(xin), ptr(c.data))
+    if c.data.is_contiguous():
+        _lib.call("surfhip_s_analyze", ws._plan, ptr(xin), ptr(c.data))
+    else:
+        # A strided view (e.g. big[::2]) is written through a packed buffer,
+        # never as if it were packed.
+        tmp = torch.empty(c.data.numel(), dtype=ws.dtype, device=ws.device)
+        _lib.call("surfhip_s_analyze", ws._plan, ptr(xin), ptr(tmp))
+        c.data.copy_(tmp)
     return c
 
 
@@ -363,12 +372,13 @@ def synthesize_(x, t, c, *, workspace=None, threaded=None):
     check_device(x, "volume")
     ws = _workspace_for(workspace, t, _eltype(x), _check_volume(x.shape), x.device)
     _check_coefficients(c, ws)
+    cd = c.data.contiguous()
     ws.bind_stream()
     if is_colmajor(x) and x.dtype == ws.dtype:
-        _lib.call("surfhip_s_synthesize", ws._plan, ptr(c.data), ptr(x))
+        _lib.call("surfhip_s_synthesize", ws._plan, ptr(cd), ptr(x))
     else:
         y = empty_colmajor(ws.dims, ws.dtype, ws.device)
-        _lib.call("surfhip_s_synthesize", ws._plan, ptr(c.data), ptr(y))
+        _lib.call("surfhip_s_synthesize", ws._plan, ptr(cd), ptr(y))
         x.copy_(y)
     return x
 
@@ -389,9 +399,10 @@ def adjoint(t, c, *, workspace=None):
     dims = workspace.dims if workspace is not None else c.layout.signal_size
     ws = _workspace_for(workspace, t, c.dtype, dims, c.device)
     _check_coefficients(c, ws)
+    cd = c.data.contiguous()
     x = empty_colmajor(ws.dims, ws.dtype, ws.device)
     ws.bind_stream()
-    _lib.call("surfhip_s_adjoint", ws._plan, ptr(c.data), ptr(x))
+    _lib.call("surfhip_s_adjoint", ws._plan, ptr(cd), ptr(x))
     return x
 
 

@@ -109,3 +109,16 @@ def test_critical_sampling_per_hourglass():
     red = lambda L: S.surfacelet_redundancy(S.Surfacelet(scales=2, direction_levels=L, checkerboard_order=4), (96, 96, 96))
     assert red([1, 1]) == red([3, 2]) == red([4, 4])
     assert abs(red([1, 1]) - (3 + 3 * (64 / 96) ** 3 + (32 / 96) ** 3)) < 1e-12
+
+
+def test_coefficients_on_the_host_are_refused_before_any_launch():
+    # A host-resident coefficient vector must never reach the kernels as a
+    # pointer: the check raises before the workspace or the library is used.
+    import types
+    import torch
+    t = S.Surfacelet(scales=1, direction_levels=[1])
+    lay = S.SubbandLayout(S._sizes(t, (24, 24, 24)), (24, 24, 24))
+    ws = types.SimpleNamespace(layout=lay, dtype=torch.float32, transform=t, dims=(24, 24, 24))
+    c = S.SubbandVector(torch.zeros(lay.length), lay)
+    with pytest.raises(surfhip.ArgumentError, match="device"):
+        S._check_coefficients(c, ws)

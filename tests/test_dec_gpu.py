@@ -204,3 +204,22 @@ def test_surfacelet_graph_replays_match_direct(dims, J, L):
         a.copy_(S.adjoint(t, c, workspace=ws))
         torch.cuda.synchronize()
         assert torch.equal(a, a0), k
+
+
+def test_strided_coefficient_vectors_are_not_read_as_packed():
+    # A strided 1-D view (every other element of a larger vector) passes the
+    # length check; analyze! must write through it and synthesize!/adjoint
+    # must read it as the view it is, leaving the rest of the parent alone.
+    dev = _dev()
+    t = S.Surfacelet(scales=2, direction_levels=[1, 2])
+    x = from_numpy(np.random.default_rng(105).standard_normal((48, 24, 36)).astype(np.float32), dev)
+    ws = S.make_workspace(t, x)
+    ref = S.analyze(t, x, workspace=ws)
+    big = torch.full((2 * len(ref),), 7.0, device=dev)
+    cs = S.SubbandVector(big[::2], ws.layout)
+    S.analyze_(cs, t, x, workspace=ws)
+    torch.cuda.synchronize()
+    assert torch.equal(big[::2], ref.data)
+    assert torch.all(big[1::2] == 7.0)
+    assert torch.equal(S.synthesize(t, cs, workspace=ws), S.synthesize(t, ref, workspace=ws))
+    assert torch.equal(S.adjoint(t, cs, workspace=ws), S.adjoint(t, ref, workspace=ws))
