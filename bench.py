@@ -29,14 +29,44 @@ sys.path.insert(0, os.path.join(ROOT, "surfacelets.jl_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch (read + written, averaged over the class's launches
-# in one step) of the plane kernels, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE passes over this bench, each scaled by what the counter reads for
-# a plain 256 MiB device copy (profiles/ns128_hbm_bytes.md).  Measured once for
-# the current kernels; not measured live.
-TRAFFIC = {"plane_forward": 231427566, "plane_inverse": 281734204}
+# HBM bytes per launch (read + written, averaged over the class's launches in
+# one step) come from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over
+# this bench, each scaled by what the counter reads for a plain 256 MiB device
+# copy.  The per-class table of that summary is read at run time, so a
+# re-measured summary is what the bench line carries.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ns128_hbm_bytes.md")
 DIMS = (128, 128, 128)
 SEED = 20260
+
+
+def traffic_per_launch(path=TRAFFIC_FILE):
+    """{class: HBM bytes per launch} from the per-class table of a counter
+    summary (rows '| class | launches | read MiB | written MiB | bytes per
+    launch | ratio |'); {} when the file is absent."""
+    out = {}
+    try:
+        with open(path) as fh:
+            for ln in fh:
+                f = [v.strip() for v in ln.strip().strip("|").split("|")]
+                if len(f) == 6 and f[0] and f[0][0].isalpha() and f[1].isdigit():
+                    try:
+                        out[f[0]] = float(f[4])
+                    except ValueError:
+                        pass
+    except OSError:
+        pass
+    return out
+
+
+def step_stats(ms_list):
+    """Median, p10, p90 and mean of per-step times (ms)."""
+    v = sorted(ms_list)
+
+    def q(f):
+        x = f * (len(v) - 1)
+        i = int(x)
+        return v[i] if i + 1 >= len(v) else v[i] + (x - i) * (v[i + 1] - v[i])
+    return {"median_ms": q(0.5), "p10_ms": q(0.1), "p90_ms": q(0.9), "mean_ms": sum(v) / len(v), "steps": len(v)}
 
 
 def parse():
@@ -75,6 +105,8 @@ def copy_calibration(mib):
 def cpu_baseline(x_np, budget_s, dims=DIMS):
     from oracle import ns as O
     from oracle.cpu_schedule import CPUSchedule
+    # The box gives one GPU's job a share of 16 host cores, whatever
+    # os.cpu_count() shows; the worker count used is reported beside it.
     workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     workers = max(1, min(workers, 16))
     t = O.NSParams(3, (2, 2, 3))
@@ -94,7 +126,8 @@ def cpu_baseline(x_np, budget_s, dims=DIMS):
     dt = (time.perf_counter() - t0) / pairs
     N = int(np.prod(dims))
     shape = "x".join(str(d) for d in dims)
-    return {"value": N * S / dt, "unit": "voxel·subbands/s", "cores": workers, "kind": "port",
+    return {"value": N * S / dt, "unit": "voxel·subbands/s", "cores": workers, "os_cpu_count": os.cpu_count(),
+            "kind": "port",
             "sample": f"{pairs} analyze+synthesize pairs of the same {shape} workload (FP32 scipy.fft, "
                       f"{workers} worker threads, one single-threaded FFT per task), {dt * 1e3:.0f} ms per pair"}
 
@@ -113,7 +146,7 @@ def decimated_cpu_baseline(t_gpu, x_np, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = (time.perf_counter() - t0) / pairs
-    return {"value": 1.0 / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+    return {"value": 1.0 / dt, "unit": "pairs/s", "cores": 1, "os_cpu_count": os.cpu_count(), "kind": "port",
             "sample": f"{pairs} analyze+synthesize pairs of the same 128^3 workload (FP64 NumPy oracle, "
                       f"one thread), {dt * 1e3:.0f} ms per pair"}
 
@@ -230,15 +263,18 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # The plan's kernels run on torch's current stream (bind_stream), so
+    # events recorded on it bracket them.  One event between consecutive
+    # steps gives per-step times; the headline is their total over K steps.
     stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.steps):
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    evs[0].record(stream)
+    for i in range(args.steps):
         step()
-    ev1.record(stream)
+        evs[i + 1].record(stream)
     torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / args.steps
+    per_step = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
+    ms = evs[0].elapsed_time(evs[-1]) / args.steps
     rel = float(torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x))
 
     # Instrumented loop: per-class kernel times from events around each launch.
@@ -256,6 +292,7 @@ def main():
         alg = {"plane_inverse": 4 * N * (S + 1), "plane_forward": 4 * N * (S + 1)}
     else:
         alg = {"rows_c2r": 4 * N * (S + 1), "rows_r2c": 4 * N * (S + 1)}
+    TRAFFIC = traffic_per_launch()
     dom = max(cls, key=lambda k: cls[k][1])
     dom_plane = max(alg, key=lambda k: cls.get(k, (0, 0.0))[1])
     launches, tms = cls[dom_plane]
@@ -274,6 +311,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "step_ms": step_stats(per_step),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -285,6 +323,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": dom_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (dom_achieved / HBM_PEAK_GBS) if dom_achieved else None,
                      "traffic": TRAFFIC.get(dom_plane) if headline else None,
+                     "traffic_source": "profiles/ns128_hbm_bytes.md" if headline and dom_plane in TRAFFIC else None,
                      "kernel": dom_plane, "launches_per_step": launches / args.steps,
                      "avg_launch_us": tms / launches * 1e3 if launches else None,
                      "dominant_class": dom},

@@ -98,10 +98,13 @@ const char *surfhip_version(void);
 int surfhip_ns_nsubbands(const surfhip_ns_params *p, int32_t *out);
 
 /* make_workspace: src/transforms/ns_surfacelet.jl:88-123,165-215.  Builds
- * the FP64 tables on the host (rounded once to FP32), uploads them and
- * allocates the plan's buffers on the current device.  `stream` is a
- * hipStream_t (NULL: the default stream).  Errors as the reference:
- * "volume size must be positive", "too small for J scales". */
+ * the one- and two-dimensional FP64 tables (filters, wedge planes, pyramid
+ * profiles) on the host and uploads them rounded once to FP32; the
+ * volume-sized group-factor tables are then built on the device in FP64
+ * from those inputs and rounded once to FP32.  Allocates the plan's buffers
+ * on the current device.  `stream` is a hipStream_t (NULL: the default
+ * stream).  Errors as the reference: "volume size must be positive", "too
+ * small for J scales"; SURFHIP_ERR_OOM when a device allocation fails. */
 int surfhip_ns_plan_create(surfhip_ns_plan **out, const surfhip_ns_params *p,
                            int32_t n1, int32_t n2, int32_t n3, int32_t dtype,
                            void *stream);
@@ -225,6 +228,32 @@ int surfhip_s_set_stream(surfhip_s_plan *plan, void *stream);
 /* analyze! / synthesize!: src/transforms/surfacelet.jl:446-464. */
 int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs);
 int surfhip_s_synthesize(surfhip_s_plan *plan, const void *coeffs, void *x);
+
+/* Integer index maps of a plan, read back from the device or computed there
+ * by the same device functions the transform kernels use (no counterpart
+ * function in the reference: an inspection hook so that the reference's
+ * integer arithmetic can be checked bit for bit).  `level` is the pyramid
+ * level and scale index, finest first (0-based); axis and hourglass K are
+ * 0-based.
+ *   kind 0: the box bins idx of level `level` along `axis`, 0-based
+ *           (src/pyramid/fourier_pyramid.jl:68-85: [0:P; n-P:n-1]);
+ *   kind 1: their lowpass bins idx_low ([0:P; n_low-P:n_low-1]);
+ *   kind 2: for each lowpass bin l along `axis`, the box coordinates the
+ *           split gathers from, two values per bin, -1 when absent (two
+ *           when 2P = n_low, as src/pyramid/fourier_pyramid.jl:108-118
+ *           assigns the later one);
+ *   kind 3: the Hermitian mirror of hourglass `hourglass` at scale `level`
+ *           along `axis` (src/directional/surfacelet_scale.jl:246-257,
+ *           mod(1 - i, n) + 1): for each index j, the index it reads, -1 for
+ *           j on the direct half of axis K;
+ *   kind 4: the flat element offset of each block of hourglass `hourglass`
+ *           at scale `level`, in column-major block order
+ *           (src/directional/surfacelet_scale.jl:57-60,
+ *           src/transforms/surfacelet.jl:78-81).
+ * Writes min(cap, length) values to out (out may be NULL) and the map's
+ * length to *count.  Synchronises the plan's stream for kinds 2 and 3. */
+int surfhip_s_index_maps(surfhip_s_plan *plan, int32_t kind, int32_t level, int32_t hourglass,
+                         int32_t axis, int64_t *out, int64_t cap, int64_t *count);
 
 /* x = A^T c, the adjoint of analyze!.  The decimated transform is a frame but
  * not a tight one (src/transforms/surfacelet.jl:24-26), so this differs from

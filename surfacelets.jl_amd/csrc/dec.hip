@@ -760,6 +760,15 @@ struct HermSum {
   float c[3];
 };
 
+// Source index along one axis of the Hermitian mirror X[-w] += c M conj(S[w])
+// (src/directional/surfacelet_scale.jl:246-257, whose mirror of 1-based i is
+// mod(1 - i, n) + 1): on axis K an element j >= n/2 + 1 reads n - j, and -1
+// marks j on the direct half; on the other axes every j reads (n - j) mod n.
+__host__ __device__ __forceinline__ int herm_mirror_src(int j, int n, bool onK) {
+  if (onK) return j < n / 2 + 1 ? -1 : n - j;
+  return (n - j) % n;
+}
+
 __global__ void k_hermitian_sum(float2 *__restrict__ X, int n1, int n2, int n3, HermSum H, int mirror) {
   const int n[3] = {n1, n2, n3};
   const int64_t tot = (int64_t)n1 * n2 * n3;
@@ -782,7 +791,7 @@ __global__ void k_hermitian_sum(float2 *__restrict__ X, int n1, int n2, int n3, 
         acc = make_float2(acc.x + w * v.x, acc.y + w * v.y);
       } else if (mirror) {
         int i[3];
-        for (int d = 0; d < 3; ++d) i[d] = d == K ? n[K] - j[K] : (n[d] - j[d]) % n[d];
+        for (int d = 0; d < 3; ++d) i[d] = herm_mirror_src(j[d], n[d], d == K);
         const int64_t s = lin3(i[0], i[1], i[2], h[0], h[1]);
         const float w = H.c[K] * H.M[K][s];
         const float2 v = H.S[K][s];
@@ -790,6 +799,23 @@ __global__ void k_hermitian_sum(float2 *__restrict__ X, int n1, int n2, int n3, 
       }
     }
     X[e] = acc;
+  }
+}
+
+// Integer maps for inspection (surfhip_s_index_maps), from the same device
+// functions the transform kernels use: kind 2, the box coordinates
+// box_coords gives each lowpass bin (two per bin, -1 when absent); kind 3,
+// herm_mirror_src along one axis.
+__global__ void k_index_map(int kind, int n, int nl, int P, int onK, int64_t *__restrict__ out) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    if (kind == 2) {
+      int c[2];
+      const int k = box_coords(e, nl, P, c);
+      out[2 * (int64_t)e] = k > 0 ? c[0] : -1;
+      out[2 * (int64_t)e + 1] = k > 1 ? c[1] : -1;
+    } else {
+      out[e] = herm_mirror_src(e, n, onK != 0);
+    }
   }
 }
 
@@ -1537,9 +1563,67 @@ int surfhip_s_set_stream(surfhip_s_plan *plan, void *stream) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+// Extents a call reads or writes through caller pointers: the volume and the
+// flat coefficient vector (complex data: two floats per element).
+void check_dec_buffers(const surfhip_s_plan *plan, const void *x, const void *coeffs) {
+  const size_t esz = (size_t)plan->nch * sizeof(float);
+  const Dims3 &d = plan->T.dims;
+  check_device_buffer(x, (size_t)d[0] * d[1] * d[2] * esz, "volume");
+  check_device_buffer(coeffs, (size_t)plan->T.ncoef * esz, "coefficient vector");
+}
+}  // namespace
+
+extern "C" {
+
+int surfhip_s_index_maps(surfhip_s_plan *plan, int32_t kind, int32_t level, int32_t hourglass, int32_t axis,
+                         int64_t *out, int64_t cap, int64_t *count) {
+  return guarded([&] {
+    if (!plan || !count) fail(SURFHIP_ERR_ARG, "null argument");
+    if (axis < 0 || axis > 2) fail(SURFHIP_ERR_ARG, fmt("axis %d outside 0:2", axis));
+    const int nlev = (int)plan->lev.size();
+    if (level < 0 || level >= nlev) fail(SURFHIP_ERR_ARG, fmt("level %d outside 0:%d", level, nlev - 1));
+    if (kind < 0 || kind > 4) fail(SURFHIP_ERR_ARG, fmt("unknown map kind %d", kind));
+    if ((kind == 3 || kind == 4) && (hourglass < 0 || hourglass > 2))
+      fail(SURFHIP_ERR_ARG, fmt("hourglass %d outside 0:2", hourglass));
+    const PyrDev &pd = plan->lev[level]->pd;
+    const int len[3] = {pd.len1, pd.len2, pd.len3};
+    const int nl[3] = {pd.l1, pd.l2, pd.l3};
+    const int off = axis == 0 ? 0 : (axis == 1 ? len[0] : len[0] + len[1]);
+    std::vector<int64_t> host;
+    if (kind == 0 || kind == 1) {
+      std::vector<int> v(len[axis]);
+      const int *src = (kind == 0 ? plan->lev[level]->idx.p : plan->lev[level]->idx_low.p) + off;
+      SH_HIP(hipMemcpy(v.data(), src, sizeof(int) * v.size(), hipMemcpyDeviceToHost));
+      host.assign(v.begin(), v.end());
+    } else if (kind == 4) {
+      const DevBuf<int64_t> &b = plan->sc[level]->boff[hourglass];
+      host.resize(b.n);
+      if (b.n) SH_HIP(hipMemcpy(host.data(), b.p, sizeof(int64_t) * b.n, hipMemcpyDeviceToHost));
+    } else {
+      const Dims3 &n = plan->T.scales[level].n;
+      const int m = kind == 2 ? nl[axis] : n[axis];
+      const int64_t total = kind == 2 ? 2 * (int64_t)m : m;
+      DevBuf<int64_t> d;
+      d.alloc((size_t)total);
+      hipLaunchKernelGGL(k_index_map, dim3((unsigned)grid_for(m)), dim3(DT), 0, plan->stream, kind, m, nl[axis],
+                         (len[axis] - 1) / 2, axis == hourglass ? 1 : 0, d.p);
+      SH_LAUNCHED();
+      host.resize((size_t)total);
+      SH_HIP(hipMemcpyAsync(host.data(), d.p, sizeof(int64_t) * total, hipMemcpyDeviceToHost, plan->stream));
+      SH_HIP(hipStreamSynchronize(plan->stream));
+    }
+    *count = (int64_t)host.size();
+    if (out) std::copy(host.begin(), host.begin() + (ptrdiff_t)std::min<int64_t>(cap, *count), out);
+  });
+}
+
 int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    check_dec_buffers(plan, x, coeffs);
     plan->run(0, x, coeffs, [&] {
       for (int ch = 0; ch < plan->nch; ++ch) plan->analyze_channel((const float *)x + ch, (float *)coeffs + ch);
     });
@@ -1549,6 +1633,7 @@ int surfhip_s_analyze(surfhip_s_plan *plan, const void *x, void *coeffs) {
 int surfhip_s_adjoint(surfhip_s_plan *plan, const void *coeffs, void *x) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    check_dec_buffers(plan, x, coeffs);
     plan->run(2, coeffs, x, [&] {
       for (int ch = 0; ch < plan->nch; ++ch) plan->adjoint_channel((const float *)coeffs + ch, (float *)x + ch);
     });
@@ -1558,6 +1643,7 @@ int surfhip_s_adjoint(surfhip_s_plan *plan, const void *coeffs, void *x) {
 int surfhip_s_synthesize(surfhip_s_plan *plan, const void *coeffs, void *x) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    check_dec_buffers(plan, x, coeffs);
     plan->run(1, coeffs, x, [&] {
       for (int ch = 0; ch < plan->nch; ++ch) plan->synthesize_channel((const float *)coeffs + ch, (float *)x + ch);
     });

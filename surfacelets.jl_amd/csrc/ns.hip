@@ -1432,6 +1432,10 @@ void create_plan(surfhip_ns_plan **out, const surfhip_ns_params *p, int32_t n1, 
       // launches of many blocks, so the last wave of blocks costs little.
       const int64_t fit = ((int64_t)1 << 30) / ((int64_t)sizeof(float2) * P->H);
       P->nwork = (int)std::max<int64_t>(1, std::min<int64_t>(P->T.S, std::max<int64_t>(8, fit)));
+      // SURFHIP_NS_NWORK caps the batch (at least 1): small volumes then take
+      // the several-batch schedule that large ones (256^3: 15 per batch) use.
+      if (const char *cap = std::getenv("SURFHIP_NS_NWORK"))
+        P->nwork = std::max(1, std::min(P->nwork, std::atoi(cap)));
       P->work.alloc((size_t)P->H * P->nwork);
     }
     // The group-table inputs are no longer needed on the host.
@@ -1464,6 +1468,15 @@ void mask_to_host(const NSTables &T, int32_t i, float *out) {
     for (int k2 = 0; k2 < T.n2; ++k2)
       for (int k1 = 0; k1 < T.h1; ++k1)
         out[(size_t)k1 + (size_t)T.h1 * ((size_t)k2 + (size_t)T.n2 * k3)] = ns_mask(v, it, k1, k2, k3, true);
+}
+
+// Extents the plan reads or writes through caller pointers: one volume, or
+// the S subbands (complex data: two floats per element).
+void check_volume_buffer(const surfhip_ns_plan *plan, const void *p, const char *what) {
+  check_device_buffer(p, (size_t)plan->N * plan->nch * sizeof(float), what);
+}
+void check_coefficient_buffer(const surfhip_ns_plan *plan, const void *p) {
+  check_device_buffer(p, (size_t)plan->N * plan->T.S * plan->nch * sizeof(float), "coefficient array");
 }
 
 }  // namespace
@@ -1554,6 +1567,8 @@ int surfhip_ns_set_stream(surfhip_ns_plan *plan, void *stream) {
 int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    check_volume_buffer(plan, x, "volume");
+    check_coefficient_buffer(plan, coeffs);
     plan->analyze((const float *)x, (float *)coeffs);
   });
 }
@@ -1561,6 +1576,8 @@ int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs) {
 int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    check_volume_buffer(plan, x, "volume");
+    check_coefficient_buffer(plan, coeffs);
     plan->synthesize((const float *)coeffs, (float *)x);
   });
 }
@@ -1568,6 +1585,8 @@ int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x) {
 int surfhip_ns_adjoint(surfhip_ns_plan *plan, const void *coeffs, void *x) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    check_volume_buffer(plan, x, "volume");
+    check_coefficient_buffer(plan, coeffs);
     with_masks(plan, 1, 0, [&] { plan->synthesize((const float *)coeffs, (float *)x); });
   });
 }
@@ -1575,6 +1594,8 @@ int surfhip_ns_adjoint(surfhip_ns_plan *plan, const void *coeffs, void *x) {
 int surfhip_ns_synthesize_adjoint(surfhip_ns_plan *plan, const void *x, void *coeffs) {
   return guarded([&] {
     if (!plan || !x || !coeffs) fail(SURFHIP_ERR_ARG, "null argument");
+    check_volume_buffer(plan, x, "volume");
+    check_coefficient_buffer(plan, coeffs);
     with_masks(plan, 0, 1, [&] { plan->analyze((const float *)x, (float *)coeffs); });
   });
 }
@@ -1582,6 +1603,7 @@ int surfhip_ns_synthesize_adjoint(surfhip_ns_plan *plan, const void *x, void *co
 int surfhip_ns_stream_begin(surfhip_ns_plan *plan, const void *x) {
   return guarded([&] {
     if (!plan || !x) fail(SURFHIP_ERR_ARG, "null argument");
+    check_volume_buffer(plan, x, "volume");
     plan->forward_spectrum((const float *)x);
     plan->streaming_fwd = true;
   });
@@ -1592,6 +1614,7 @@ int surfhip_ns_stream_subband(surfhip_ns_plan *plan, int32_t i, void *band) {
     if (!plan || !band) fail(SURFHIP_ERR_ARG, "null argument");
     if (!plan->streaming_fwd) fail(SURFHIP_ERR_ARG, "surfhip_ns_stream_begin was not called");
     if (i < 1 || i > plan->T.S) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", i, plan->T.S));
+    check_volume_buffer(plan, band, "subband buffer");
     plan->stream_subband(i - 1, (float *)band);
   });
 }
@@ -1607,6 +1630,7 @@ int surfhip_ns_stream_accumulate(surfhip_ns_plan *plan, int32_t i, const void *b
   return guarded([&] {
     if (!plan || !band) fail(SURFHIP_ERR_ARG, "null argument");
     if (i < 1 || i > plan->T.S) fail(SURFHIP_ERR_ARG, fmt("subband index %d outside 1:%d", i, plan->T.S));
+    check_volume_buffer(plan, band, "subband buffer");
     plan->accumulate(i - 1, (const float *)band);
   });
 }
@@ -1614,6 +1638,7 @@ int surfhip_ns_stream_accumulate(surfhip_ns_plan *plan, int32_t i, const void *b
 int surfhip_ns_stream_finish(surfhip_ns_plan *plan, void *x) {
   return guarded([&] {
     if (!plan || !x) fail(SURFHIP_ERR_ARG, "null argument");
+    check_volume_buffer(plan, x, "volume");
     plan->finish((float *)x);
   });
 }

@@ -16,15 +16,6 @@ from .directional import (hourglass_decompose, hourglass_decompose_, hourglass_m
                           ndfb_sharing, ndfb_subband_index, ndfb_subband_triple, ndfb_weight)
 
 
-def _dispatch(name, t, *args, **kw):
-    mod = _ns
-    from . import decimated as _dec
-    target = t
-    if isinstance(t, (_dec.Surfacelet, _dec.SurfaceletWorkspace)):
-        mod = _dec
-    return getattr(mod, name)(*args, **kw)
-
-
 def _is_dec(t):
     from . import decimated as _dec
     return isinstance(t, (_dec.Surfacelet, _dec.SurfaceletWorkspace))

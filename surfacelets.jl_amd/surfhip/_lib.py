@@ -28,6 +28,10 @@ class DeviceError(RuntimeError):
     """A HIP runtime failure or a device allocation failure."""
 
 
+class DeviceOutOfMemory(DeviceError, MemoryError):
+    """A device allocation the GPU cannot satisfy (SURFHIP_ERR_OOM)."""
+
+
 class NSParams(ctypes.Structure):
     _fields_ = [
         ("scales", ctypes.c_int32),
@@ -88,6 +92,8 @@ _SIGS = {
     "surfhip_s_analyze": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_s_synthesize": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_s_adjoint": (ctypes.c_int, [_P, _P, _P]),
+    "surfhip_s_index_maps": (ctypes.c_int, [_P, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _lock = threading.Lock()
@@ -129,6 +135,8 @@ def check(status):
         raise ArgumentError(msg)
     if status == ERR_DIM:
         raise DimensionMismatch(msg)
+    if status == ERR_OOM:
+        raise DeviceOutOfMemory(msg)
     raise DeviceError(msg)
 
 

@@ -406,6 +406,27 @@ def adjoint(t, c, *, workspace=None):
     return x
 
 
+INDEX_MAP_KINDS = {"box": 0, "box_low": 1, "split_gather": 2, "mirror": 3, "block_offsets": 4}
+
+
+def index_map(ws, kind, level, *, axis=0, hourglass=0):
+    """An integer index map of the workspace's plan as the device holds or
+    computes it (surfhip_s_index_maps; level, axis and hourglass 0-based,
+    finest level first): 'box' / 'box_low' the pyramid box bins and their
+    lowpass bins along `axis`; 'split_gather' the box coordinates each
+    lowpass bin gathers from, shape (n_low, 2), -1 when absent; 'mirror' the
+    Hermitian mirror index along `axis` of `hourglass` at scale `level`, -1
+    on the direct half; 'block_offsets' the flat offset of each block of
+    `hourglass` at scale `level`."""
+    k = INDEX_MAP_KINDS[kind]
+    n = ctypes.c_int64()
+    _lib.call("surfhip_s_index_maps", ws._plan, k, level, hourglass, axis, None, 0, ctypes.byref(n))
+    buf = (ctypes.c_int64 * max(1, n.value))()
+    _lib.call("surfhip_s_index_maps", ws._plan, k, level, hourglass, axis, buf, n.value, ctypes.byref(n))
+    out = np.frombuffer(buf, dtype=np.int64, count=n.value).copy()
+    return out.reshape(-1, 2) if k == 2 else out
+
+
 def check_size(t, dims):
     """Raise the reference's ArgumentError (with the nearest fitting length)
     when `dims` does not fit `t` (src/transforms/surfacelet.jl:253-269)."""

@@ -223,3 +223,58 @@ def test_strided_coefficient_vectors_are_not_read_as_packed():
     assert torch.all(big[1::2] == 7.0)
     assert torch.equal(S.synthesize(t, cs, workspace=ws), S.synthesize(t, ref, workspace=ws))
     assert torch.equal(S.adjoint(t, cs, workspace=ws), S.adjoint(t, ref, workspace=ws))
+
+
+@pytest.mark.parametrize("dims,mode,J,L", [
+    ((128, 128, 128), 2, 3, [2, 2, 3]),
+    ((96, 96, 96), 1.5, 2, [L212, L111]),
+    ((48, 24, 36), 1.5, 1, [2]),
+    ((64, 32, 32), 2, 2, [1, 3]),
+])
+def test_integer_index_maps_match_oracle_bitwise(dims, mode, J, L):
+    # The plan's integer maps, read back from the device or computed there by
+    # the kernels' own index functions (surfhip_s_index_maps), against the
+    # oracle's: pyramid boxes and their lowpass bins
+    # (src/pyramid/fourier_pyramid.jl:68-85), the split's gather map with the
+    # later-element rule where 2P = n_low (:108-118), the Hermitian mirror
+    # mod(1 - i, n) + 1 (src/directional/surfacelet_scale.jl:246-257), and
+    # the flat offset of every block in column-major block order
+    # (src/directional/surfacelet_scale.jl:57-60).
+    _dev()
+    t = S.Surfacelet(scales=J, direction_levels=L, pyramid_ratio=mode)
+    to = D.SParams(scales=J, direction_levels=tuple(L), pyramid_ratio=mode)
+    ws = S.make_workspace(t, torch.float32, dims)
+    wo = D.workspace(to, dims)
+    sizes = D.layout_sizes(to, dims)
+    elem_off = np.concatenate([[0], np.cumsum([int(np.prod(s)) for s in sizes])])
+    shared = 0
+    for lev, pl in enumerate(wo.pyramid):
+        for a in range(3):
+            box = S.index_map(ws, "box", lev, axis=a)
+            low = S.index_map(ws, "box_low", lev, axis=a)
+            assert np.array_equal(box, pl.idx[a]), (lev, a)
+            assert np.array_equal(low, pl.idx_low[a]), (lev, a)
+            gather = S.index_map(ws, "split_gather", lev, axis=a)
+            assert gather.shape == (pl.nlow[a], 2)
+            for lb in range(pl.nlow[a]):
+                want = [int(c) for c in np.nonzero(pl.idx_low[a] == lb)[0]]
+                got = [int(c) for c in gather[lb] if c >= 0]
+                assert got == want, (lev, a, lb)
+                shared += len(want) == 2
+    for j, sc in enumerate(wo.scales):
+        first = wo.offsets[j]
+        for K in range(3):
+            for a in range(3):
+                n = sc.dims[a]
+                h = n // 2 + 1
+                m = S.index_map(ws, "mirror", j, axis=a, hourglass=K)
+                jj = np.arange(n)
+                want = np.where(jj < h, -1, n - jj) if a == K else (-jj) % n
+                assert np.array_equal(m, want), (j, K, a)
+            counts, _ = D.surf_blocks(sc, K)
+            nb = int(np.prod(counts))
+            offs = S.index_map(ws, "block_offsets", j, hourglass=K)
+            assert np.array_equal(offs, elem_off[first:first + nb]), (j, K)
+            first += nb
+    if dims == (128, 128, 128):
+        assert shared > 0   # the 2P = n_low levels exercise the later-element rule
