@@ -125,6 +125,19 @@ int surfhip_ns_plan_tune(surfhip_ns_plan *plan, int32_t *config);
  * coeffs: (n1, n2, n3, S). */
 int surfhip_ns_analyze(surfhip_ns_plan *plan, const void *x, void *coeffs);
 
+/* analyze! that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * plan's analysis runs with its working spectra and slabs in a static device
+ * array of 64 MiB per GPU instead of the plan's own buffers; the
+ * coefficients equal surfhip_ns_analyze's bit for bit.  A plan whose
+ * analysis needs more working memory (e.g. 128^3 on the factored kernels,
+ * 179 MB) is refused with SURFHIP_ERR_ARG.  All calls on one GPU share that
+ * array: calls that overlap in time on one GPU race.  The first call on a
+ * device looks up the array's address (loading it); later calls allocate
+ * nothing. */
+int surfhip_ns_analyze_noalloc(surfhip_ns_plan *plan, const void *x, void *coeffs);
+
 /* synthesize!: src/transforms/ns_surfacelet.jl:568-581. */
 int surfhip_ns_synthesize(surfhip_ns_plan *plan, const void *coeffs, void *x);
 

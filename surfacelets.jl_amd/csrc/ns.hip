@@ -25,6 +25,7 @@
 #include "fft.hpp"
 #include "hip_util.hpp"
 #include "ns_fused.hpp"
+#include "ns_scratch.hpp"
 #include "ns_tables.hpp"
 
 namespace surfhip {
@@ -1170,6 +1171,58 @@ struct surfhip_ns_plan {
     }
     cached_group = -1;
   }
+  // Working memory of analyze() carved from one region: pieces of `need`
+  // bytes each, 256-byte aligned.
+  static size_t aligned(size_t b) { return (b + 255) & ~(size_t)255; }
+  size_t working_bytes(int work_spectra) const {
+    const size_t spec = aligned(sizeof(float2) * (size_t)H);
+    if (fused) return spec * (1 + nch) + aligned(sizeof(float2) * (size_t)nslabs * H);
+    return spec * nch + aligned(sizeof(float2) * (size_t)H * work_spectra);
+  }
+  // analyze() with TV, X, U (factored) or X and work (general) pointing into
+  // [ws, ws + bytes); the plan's own buffers are put back afterwards.  The
+  // general path's batch shrinks to the work spectra that fit, which changes
+  // no coefficient (each subband's arithmetic is independent of its batch).
+  // Channels share the slabs: each channel's analysis joins both
+  // orientations before the next starts.
+  void analyze_in(char *ws, size_t bytes, const float *x, float *coeffs) {
+    struct Saved {
+      surfhip_ns_plan *p;
+      float2 *tv, *x0, *x1, *u0, *u1, *work;
+      int nwork;
+      ~Saved() {
+        p->TV.p = tv;
+        p->X[0].p = x0;
+        p->X[1].p = x1;
+        p->U[0].p = u0;
+        p->U[1].p = u1;
+        p->work.p = work;
+        p->nwork = nwork;
+      }
+    } saved{this, TV.p, X[0].p, X[1].p, U[0].p, U[1].p, work.p, nwork};
+    const size_t spec = aligned(sizeof(float2) * (size_t)H);
+    char *at = ws;
+    auto take = [&](size_t b) {
+      char *q = at;
+      at += aligned(b);
+      if ((size_t)(at - ws) > bytes) fail(SURFHIP_ERR_ARG, "working region too small");
+      return reinterpret_cast<float2 *>(q);
+    };
+    if (fused) {
+      TV.p = take(spec);
+      for (int ch = 0; ch < nch; ++ch) X[ch].p = take(spec);
+      U[0].p = U[1].p = take(sizeof(float2) * (size_t)nslabs * H);
+    } else {
+      for (int ch = 0; ch < nch; ++ch) X[ch].p = take(spec);
+      const size_t left = bytes - (size_t)(at - ws);
+      const int nb = (int)std::min<size_t>((size_t)nwork, left / (sizeof(float2) * (size_t)H));
+      if (nb < 1) fail(SURFHIP_ERR_ARG, "working region too small");
+      work.p = take(sizeof(float2) * (size_t)H * nb);
+      nwork = nb;
+    }
+    analyze(x, coeffs);
+  }
+
   // Streaming analysis: subband i (0-based) into band.
   void stream_subband(int i, float *band) {
     if (!fused) {
@@ -1480,6 +1533,29 @@ void check_coefficient_buffer(const surfhip_ns_plan *plan, const void *p) {
 }
 
 }  // namespace
+
+namespace surfhip {
+
+size_t ns_analyze_working_bytes(const surfhip_ns_plan *plan) { return plan->working_bytes(1); }
+
+int ns_plan_device(const surfhip_ns_plan *plan) {
+  int dev = 0;
+  if (plan->stream) {
+    SH_HIP(hipStreamGetDevice(plan->stream, &dev));
+  } else {
+    SH_HIP(hipGetDevice(&dev));
+  }
+  return dev;
+}
+
+void ns_analyze_in(surfhip_ns_plan *plan, void *ws, size_t bytes, const void *x, void *coeffs) {
+  if (!plan || !x || !coeffs || !ws) fail(SURFHIP_ERR_ARG, "null argument");
+  check_volume_buffer(plan, x, "volume");
+  check_coefficient_buffer(plan, coeffs);
+  plan->analyze_in(static_cast<char *>(ws), bytes, (const float *)x, (float *)coeffs);
+}
+
+}  // namespace surfhip
 
 extern "C" {
 

@@ -66,6 +66,7 @@ _SIGS = {
     "surfhip_ns_set_stream": (ctypes.c_int, [_P, _P]),
     "surfhip_ns_plan_tune": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "surfhip_ns_analyze": (ctypes.c_int, [_P, _P, _P]),
+    "surfhip_ns_analyze_noalloc": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_ns_synthesize": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_ns_adjoint": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_ns_synthesize_adjoint": (ctypes.c_int, [_P, _P, _P]),
