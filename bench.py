@@ -83,9 +83,11 @@ def parse():
                     metavar="N1,N2,N3",
                     help="ns workload on another volume size (default 128,128,128, the headline); sizes off "
                          "the power-of-two grids run the general-size kernels")
-    ap.add_argument("--workload", choices=("ns", "decimated"), default="ns",
+    ap.add_argument("--workload", choices=("ns", "decimated", "batch"), default="ns",
                     help="ns: the NSSurfacelet headline (default); decimated: the decimated Surfacelet "
-                         "(scales=3, direction_levels=[2,2,3], pyramid_ratio=2) on the same volume")
+                         "(scales=3, direction_levels=[2,2,3], pyramid_ratio=2) on the same volume; batch: 64 "
+                         "volumes of uneven sizes, the batch against one volume per call")
+    ap.add_argument("--lanes", type=int, default=4, help="batch workload: streams of the batch")
     return ap.parse_args()
 
 
@@ -225,6 +227,82 @@ def bench_decimated(args):
     print(json.dumps(result), flush=True)
 
 
+BATCH_SIZES = [(64, 64, 64), (80, 96, 72), (96, 96, 96), (112, 128, 96), (128, 128, 128)]
+
+
+def bench_batch(args):
+    """64 Float32 volumes drawn uniformly (seed 400) from BATCH_SIZES,
+    ellipsoid phantoms plus N(0, 0.05^2), NSSurfacelet(scales=3,
+    direction_levels=[2,2,3]).  One step = analyze + synthesize of all 64.
+    plain: one volume per call on one stream, one plan per size; batch: the
+    library's batch (lanes, one recorded graph replayed per step).  Both are
+    timed with HIP events on the current stream, per step."""
+    global np
+    import numpy as np
+    import torch
+    from surfhip import ns as NS
+    from surfhip._arrays import from_numpy
+    from surfhip.batch import NSBatch
+    from surfhip.phantom import ellipsoid_phantom
+
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(400)
+    idx = rng.integers(0, len(BATCH_SIZES), size=64)
+    shapes = [BATCH_SIZES[i] for i in idx]
+    seeds = [int(v) for v in rng.integers(0, 2 ** 31, size=64)]
+    xs = [from_numpy(ellipsoid_phantom(s, seed=sd, noise=0.05), "cuda") for s, sd in zip(shapes, seeds)]
+    t = NS.NSSurfacelet(scales=3, direction_levels=[2, 2, 3])
+    wss = {s: NS.make_workspace(t, torch.float32, s, prewarm=False) for s in BATCH_SIZES}
+    cs = [NS.similar_coefficients(t, x) for x in xs]
+    ys = [torch.empty_like(x) for x in xs]
+    b = NSBatch(t, shapes, lanes=args.lanes)
+    bc = b.coefficients()
+    by = [torch.empty_like(x) for x in xs]
+    stream = torch.cuda.current_stream()
+
+    def plain():
+        for k, x in enumerate(xs):
+            ws = wss[shapes[k]]
+            NS.analyze_(cs[k], ws, x)
+            NS.synthesize_(ys[k], ws, cs[k])
+
+    def batched():
+        b.run(xs, bc, by)
+
+    out = {}
+    for name, fn in (("plain", plain), ("batch", batched)):
+        for _ in range(max(2, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        evs[0].record(stream)
+        for i in range(args.steps):
+            fn()
+            evs[i + 1].record(stream)
+        torch.cuda.synchronize()
+        per = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
+        out[name] = step_stats(per)
+    same = all(torch.equal(bc[k], cs[k]) and torch.equal(by[k], ys[k]) for k in range(64))
+    voxsub = sum(int(np.prod(s)) * NS.nsubbands(t, s) for s in shapes)
+    ms = out["batch"]["median_ms"]
+    result = {
+        "metric": "batched analyze+synth voxel·subbands/s, 64 uneven Float32 volumes, 49 subbands",
+        "value": voxsub / (ms * 1e-3),
+        "unit": "voxel·subbands/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": max(2, args.warmup), "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: ellipsoid phantoms + N(0, 0.05^2), sizes and phantoms from numpy PCG64 seed 400",
+        "config": {"workload": "64 volumes from {64^3, 80x96x72, 96^3, 112x128x96, 128^3}, "
+                               "NSSurfacelet(scales=3, direction_levels=[2,2,3]) analyze+synthesize",
+                   "counts": {"x".join(map(str, z)): sum(s == z for s in shapes) for z in BATCH_SIZES},
+                   "lanes": args.lanes, "parallelism": "single GPU"},
+        "plain_step_ms": out["plain"], "batch_step_ms": out["batch"],
+        "speedup_vs_plain": out["plain"]["median_ms"] / ms,
+        "check": {"batch_equals_plain_bitwise": same},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -236,6 +314,9 @@ def main():
         sys.exit(2)
     if args.workload == "decimated":
         bench_decimated(args)
+        return
+    if args.workload == "batch":
+        bench_batch(args)
         return
     global np
     import numpy as np

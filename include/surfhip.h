@@ -166,6 +166,33 @@ int surfhip_ns_stream_inverse_begin(surfhip_ns_plan *plan);
 int surfhip_ns_stream_accumulate(surfhip_ns_plan *plan, int32_t i, const void *band);
 int surfhip_ns_stream_finish(surfhip_ns_plan *plan, void *x);
 
+/* Batched transforms of many volumes of uneven sizes (no counterpart in the
+ * reference, which transforms one volume per call,
+ * src/transforms/ns_surfacelet.jl:551-581, and has no device batching,
+ * :147).  A batch holds one plan per (lane, volume size) for `nvol` volumes
+ * (dims: n1 n2 n3 per volume) and spreads the volumes over `lanes` streams,
+ * longest first onto the least-loaded lane.  surfhip_ns_batch_run with mode
+ * 0 analyzes xs[k] into cs[k], mode 1 synthesizes cs[k] into ys[k], mode 2
+ * does both, for every k; it is ordered after the batch stream's earlier work
+ * and before its later work.  A run is recorded once as a hipGraph and
+ * replayed with one launch while the mode and the pointers repeat; a batch
+ * stream that is itself being captured gets direct launches.  Results equal
+ * each volume's lone surfhip_ns_analyze / _synthesize on a plan of the same
+ * configuration, bit for bit.  flags: SURFHIP_BATCH_TUNE runs each plan's
+ * autotuner, SURFHIP_BATCH_NO_GRAPH always launches directly.
+ * surfhip_ns_batch_info returns each volume's lane and whether a graph is
+ * recorded. */
+typedef struct surfhip_ns_batch surfhip_ns_batch;
+enum { SURFHIP_BATCH_TUNE = 1, SURFHIP_BATCH_NO_GRAPH = 2 };
+int surfhip_ns_batch_create(surfhip_ns_batch **out, const surfhip_ns_params *p, int32_t nvol,
+                            const int32_t *dims, int32_t dtype, int32_t lanes, int32_t flags,
+                            void *stream);
+int surfhip_ns_batch_run(surfhip_ns_batch *b, int32_t mode, const void *const *xs,
+                         void *const *cs, void *const *ys);
+int surfhip_ns_batch_set_stream(surfhip_ns_batch *b, void *stream);
+int surfhip_ns_batch_info(surfhip_ns_batch *b, int32_t *lane_of, int32_t *graph_recorded);
+int surfhip_ns_batch_destroy(surfhip_ns_batch *b);
+
 /* Host only: the analysis mask of subband i (1-based) on the rfft grid,
  * (n1/2+1, n2, n3) column-major, as the device evaluates it (the same FP32
  * tables and expression).  The inspection counterpart of hourglass_mask /

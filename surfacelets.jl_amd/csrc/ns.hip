@@ -25,6 +25,7 @@
 #include "fft.hpp"
 #include "hip_util.hpp"
 #include "ns_fused.hpp"
+#include "ns_plan_api.hpp"
 #include "ns_scratch.hpp"
 #include "ns_tables.hpp"
 
@@ -1537,6 +1538,15 @@ void check_coefficient_buffer(const surfhip_ns_plan *plan, const void *p) {
 namespace surfhip {
 
 size_t ns_analyze_working_bytes(const surfhip_ns_plan *plan) { return plan->working_bytes(1); }
+
+size_t ns_volume_bytes(const surfhip_ns_plan *plan) { return (size_t)plan->N * plan->nch * sizeof(float); }
+size_t ns_coefficient_bytes(const surfhip_ns_plan *plan) { return ns_volume_bytes(plan) * plan->T.S; }
+void ns_analyze_raw(surfhip_ns_plan *plan, const void *x, void *coeffs) {
+  plan->analyze((const float *)x, (float *)coeffs);
+}
+void ns_synthesize_raw(surfhip_ns_plan *plan, const void *coeffs, void *x) {
+  plan->synthesize((const float *)coeffs, (float *)x);
+}
 
 int ns_plan_device(const surfhip_ns_plan *plan) {
   int dev = 0;

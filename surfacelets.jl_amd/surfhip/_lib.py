@@ -93,6 +93,12 @@ _SIGS = {
     "surfhip_s_analyze": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_s_synthesize": (ctypes.c_int, [_P, _P, _P]),
     "surfhip_s_adjoint": (ctypes.c_int, [_P, _P, _P]),
+    "surfhip_ns_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(NSParams), _I,
+                                               ctypes.POINTER(_I), _I, _I, _I, _P]),
+    "surfhip_ns_batch_run": (ctypes.c_int, [_P, _I, _P, _P, _P]),
+    "surfhip_ns_batch_set_stream": (ctypes.c_int, [_P, _P]),
+    "surfhip_ns_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "surfhip_ns_batch_destroy": (ctypes.c_int, [_P]),
     "surfhip_s_index_maps": (ctypes.c_int, [_P, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_int64)]),
 }

@@ -1,107 +1,100 @@
-"""Batched NSSurfacelet analyze+synthesize of many volumes of uneven sizes.
+"""Batched NSSurfacelet analyze / synthesize of many volumes of uneven sizes,
+on libsurfhip's batch (surfhip_ns_batch_*).
 
-The volumes are spread over `lanes` HIP streams (longest-first onto the
-least-loaded lane), each lane with its own workspace per distinct size, so
-transforms of different volumes run concurrently and fill the GPU where one
-volume's kernels would not.  `capture` records the whole batch, every lane
-forked from and joined back to the capturing stream, as one graph (hipGraph,
-through torch.cuda.CUDAGraph) that `replay` launches with a single call.
-Workspaces are autotuned by default (make_workspace(...; tune=true)).
-
-Graph capture records kernel launches only: the transforms' launch sequence
-holds no host synchronisation and no memset (buffers are cleared by
-kernels), so a replay recomputes exactly what the eager run computes.
+The library holds one plan per (lane, volume size) and spreads the volumes
+over `lanes` HIP streams, longest first onto the least-loaded lane, so
+transforms of different volumes overlap.  A run is recorded once as a
+hipGraph and replayed with a single launch while the mode and the arrays
+repeat (graph=True); results equal each volume's lone transform on a plan of
+the same configuration, bit for bit.  The reference transforms one volume per
+call and has no device batching (src/transforms/ns_surfacelet.jl:147,551-581).
 """
+import ctypes
+
 import torch
 
+from . import _lib
 from . import ns as _ns
-from ._arrays import empty_colmajor, is_colmajor
+from ._arrays import ctypes_void, empty_colmajor, is_colmajor
 from ._lib import ArgumentError
+
+TUNE, NO_GRAPH = 1, 2
+MODES = {"analyze": 0, "synthesize": 1, "both": 2}
 
 
 class NSBatch:
-    def __init__(self, t, shapes, dtype=torch.float32, device="cuda", lanes=4, tune=True):
+    def __init__(self, t, shapes, dtype=torch.float32, device="cuda", lanes=4, tune=False, graph=True):
         if lanes < 1:
             raise ArgumentError(f"lanes must be ≥ 1, got {lanes}")
         if len(shapes) == 0:
             raise ArgumentError("a batch needs at least one volume")
         self.transform = t
-        self.shapes = [tuple(int(d) for d in s) for s in shapes]
-        self.dtype = dtype
+        self.shapes = [_ns._check_volume(s) for s in shapes]
+        self.dtype = _ns._eltype(dtype)
         self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device(self.device.type, torch.cuda.current_device())
         self.lanes = min(lanes, len(self.shapes))
-        # Longest-processing-time assignment: biggest volumes first, each onto
-        # the lane with the least voxels so far.
-        order = sorted(range(len(self.shapes)), key=lambda k: -_voxels(self.shapes[k]))
-        load = [0] * self.lanes
-        self.lane_of = [0] * len(self.shapes)
-        for k in order:
-            lane = min(range(self.lanes), key=lambda q: load[q])
-            self.lane_of[k] = lane
-            load[lane] += _voxels(self.shapes[k])
-        self.streams = [torch.cuda.Stream(self.device) for _ in range(self.lanes)]
-        self.workspaces = {}
-        self.tuning = {}
-        for k, s in enumerate(self.shapes):
-            key = (self.lane_of[k], s)
-            if key not in self.workspaces:
-                ws = _ns.make_workspace(t, dtype, s, device=self.device, prewarm=False)
-                if tune:
-                    self.tuning[key] = ws.tune()
-                self.workspaces[key] = ws
+        self.nsub = [_ns.nsubbands(t, s) for s in self.shapes]
+        dims = (ctypes.c_int32 * (3 * len(self.shapes)))(*[d for s in self.shapes for d in s])
+        flags = (TUNE if tune else 0) | (0 if graph else NO_GRAPH)
+        self._b = ctypes.c_void_p()
+        code = _lib.F32 if self.dtype == torch.float32 else _lib.C64
+        with torch.cuda.device(self.device):
+            _lib.call("surfhip_ns_batch_create", ctypes.byref(self._b), ctypes.byref(t.c_params()),
+                      len(self.shapes), dims, code, lanes, flags,
+                      ctypes_void(torch.cuda.current_stream(self.device).cuda_stream))
+        lane_of = (ctypes.c_int32 * len(self.shapes))()
+        _lib.call("surfhip_ns_batch_info", self._b, lane_of, None)
+        self.lane_of = list(lane_of)
 
-    def workspace(self, k):
-        """The workspace that transforms volume k."""
-        return self.workspaces[(self.lane_of[k], self.shapes[k])]
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b is not None and b.value:
+            try:
+                _lib.lib().surfhip_ns_batch_destroy(b)
+            except Exception:
+                pass
+            self._b = None
+
+    def graph_recorded(self):
+        rec = ctypes.c_int32()
+        _lib.call("surfhip_ns_batch_info", self._b, None, ctypes.byref(rec))
+        return bool(rec.value)
 
     def coefficients(self):
         """Column-major coefficient arrays for every volume of the batch."""
-        return [_ns.similar_coefficients(self.transform, empty_colmajor(s, self.dtype, self.device))
-                for s in self.shapes]
+        return [empty_colmajor(s + (n,), self.dtype, self.device) for s, n in zip(self.shapes, self.nsub)]
 
-    def _check(self, xs, cs, ys):
-        for name, arrs in (("volumes", xs), ("coefficients", cs), ("outputs", ys)):
-            if len(arrs) != len(self.shapes):
-                raise ArgumentError(f"{len(arrs)} {name} for a batch of {len(self.shapes)}")
-        for k, s in enumerate(self.shapes):
-            for name, a, shape in (("volume", xs[k], s), ("output", ys[k], s)):
-                if tuple(a.shape) != s or not is_colmajor(a) or a.dtype != self.dtype:
-                    raise ArgumentError(f"{name} {k} must be a column-major {self.dtype} array of size {s}")
-            if not is_colmajor(cs[k]) or cs[k].dtype != self.dtype:
-                raise ArgumentError(f"coefficients {k} must be a column-major {self.dtype} array")
+    def _check(self, name, arrs, coef):
+        if arrs is None:
+            raise ArgumentError(f"{name} are needed for this mode")
+        if len(arrs) != len(self.shapes):
+            raise ArgumentError(f"{len(arrs)} {name} for a batch of {len(self.shapes)}")
+        for k, (a, s, n) in enumerate(zip(arrs, self.shapes, self.nsub)):
+            shape = s + (n,) if coef else s
+            if (not isinstance(a, torch.Tensor) or a.device != self.device or tuple(a.shape) != shape
+                    or not is_colmajor(a) or a.dtype != self.dtype):
+                raise ArgumentError(f"{name} {k} must be a column-major {self.dtype} array of size {shape} "
+                                    f"on {self.device}")
 
-    def run(self, xs, cs, ys):
-        """cs[k] = analyze(xs[k]); ys[k] = synthesize(cs[k]) for every volume,
-        the lanes running concurrently; ordered after the current stream's
-        earlier work and before its later work."""
-        self._check(xs, cs, ys)
-        main = torch.cuda.current_stream(self.device)
-        for s in self.streams:
-            s.wait_stream(main)
-        for k in range(len(self.shapes)):
-            with torch.cuda.stream(self.streams[self.lane_of[k]]):
-                ws = self.workspace(k)
-                _ns.analyze_(cs[k], ws, xs[k])
-                _ns.synthesize_(ys[k], ws, cs[k])
-        for s in self.streams:
-            main.wait_stream(s)
-        for ws in self.workspaces.values():
-            ws.bind_stream()
-        return ys
-
-    def capture(self, xs, cs, ys):
-        """Record run(xs, cs, ys) as one graph; replay it with graph.replay()."""
-        self._check(xs, cs, ys)
-        side = torch.cuda.Stream(self.device)
-        side.wait_stream(torch.cuda.current_stream(self.device))
-        with torch.cuda.stream(side):
-            self.run(xs, cs, ys)  # eager pass: first-use work stays out of the graph
-        torch.cuda.current_stream(self.device).wait_stream(side)
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=side):
-            self.run(xs, cs, ys)
-        return g
-
-
-def _voxels(s):
-    return s[0] * s[1] * s[2]
+    def run(self, xs=None, cs=None, ys=None, mode="both"):
+        """mode 'analyze': cs[k] = analyze(xs[k]); 'synthesize': ys[k] =
+        synthesize(cs[k]); 'both': the two in turn, for every volume.
+        Ordered after the current stream's earlier work and before its later
+        work."""
+        m = MODES[mode]
+        if m != 1:
+            self._check("volumes", xs, False)
+        self._check("coefficients", cs, True)
+        if m != 0:
+            self._check("outputs", ys, False)
+        n = len(self.shapes)
+        P = ctypes.c_void_p * n
+        xa = P(*[x.data_ptr() for x in xs]) if m != 1 else None
+        ca = P(*[c.data_ptr() for c in cs])
+        ya = P(*[y.data_ptr() for y in ys]) if m != 0 else None
+        _lib.call("surfhip_ns_batch_set_stream", self._b,
+                  ctypes_void(torch.cuda.current_stream(self.device).cuda_stream))
+        _lib.call("surfhip_ns_batch_run", self._b, m, xa, ca, ya)
+        return ys if m != 0 else cs
