@@ -17,10 +17,15 @@ magnitude, theta_j applies to every directional subband of scale j (scales
 1..J, coarse to fine, as in subband_layout) and the lowpass is not
 thresholded.
 
-Each iteration is one autograd node that saves only its input c_k.  Its
-forward and backward both run A through the streaming analysis, one subband
-at a time, so besides the saved iterates a step holds one coefficient-sized
-buffer and one subband:
+The unrolled loop is one autograd node with segment checkpoints: the
+forward keeps only the iterates c_s, c_2s, ... (every s = ceil(sqrt(K))
+steps, `checkpoint_every`), and the backward walks the segments from the
+last to the first, recomputing each iterate c_k from its segment's
+checkpoint when step k's backward needs it.  So at most K/s + 1 iterates
+(plus one incoming gradient and one work buffer) are resident, against K
+when every iterate is saved, for about K s / 2 extra forward steps.  Each
+step's forward and backward run A through the streaming analysis, one
+subband at a time:
   forward:  r = y - A^T c;  z_i = c_i + (A r)_i;  c'_i = soft(z_i, theta_i)
   backward: given g' = dL/dc', recompute z_i as above;
             G_i = (dsoft/dz at z_i)^T g'_i and dL/dtheta_j += <g'_i, dsoft/dtheta>
@@ -28,6 +33,8 @@ buffer and one subband:
             u = A^T G;  dL/dc = G - A u;  dL/dy = u
 (z = (I - A A^T) c + A y, so dz/dc = I - A A^T and dz/dy = A).
 """
+import math
+
 import torch
 
 from . import autograd as _ag
@@ -58,59 +65,95 @@ def _coefficients(ws):
     return empty_colmajor(tuple(ws.dims) + (ws.nsub,), ws.dtype, ws.device)
 
 
-class _ISTAStep(torch.autograd.Function):
-    """c' = soft(c + A (y - A^T c), theta[scale(i)]) subband by subband."""
+def _step(ws, scales, c, theta, y):
+    """c' = soft(c + A (y - A^T c), theta[scale(i)]) subband by subband
+    (c = None is c_0 = 0)."""
+    cn = _coefficients(ws)
+    r = y if c is None else y - _ns.synthesize(ws, c)
+
+    def f(i, band):
+        z = band if c is None else _band(c, i) + band
+        s = scales[i - 1]
+        _band(cn, i).copy_(z if s == 0 else soft_threshold(z, theta[s - 1]))
+
+    _ns.analyze(f, ws, r)
+    return cn
+
+
+def _step_backward(ws, scales, c, theta, y, gn, gtheta):
+    """(dL/dc, dL/dy) of one step given gn = dL/dc'; adds dL/dtheta into
+    gtheta.  c = None is c_0 = 0 (its gradient is not needed)."""
+    G = _coefficients(ws)
+    r = y if c is None else y - _ns.synthesize(ws, c)
+
+    def f(i, band):
+        s = scales[i - 1]
+        gi = gn[:, :, :, i - 1]
+        if s == 0:
+            _band(G, i).copy_(gi)
+            return
+        with torch.enable_grad():
+            z = (band if c is None else _band(c, i) + band).detach().requires_grad_(True)
+            th = theta[s - 1].detach().requires_grad_(True)
+            gz, gt = torch.autograd.grad(soft_threshold(z, th), (z, th), gi)
+        _band(G, i).copy_(gz)
+        gtheta[s - 1] += gt
+
+    _ns.analyze(f, ws, r)
+    u = _ns.synthesize(ws, G)
+    if c is None:
+        return None, u
+
+    def h(i, band):
+        _band(G, i).sub_(band)
+
+    _ns.analyze(h, ws, u)
+    return G, u
+
+
+class _ISTA(torch.autograd.Function):
+    """c_K of `iters` ISTA steps from c_0 = 0, differentiable in theta and y,
+    with segment checkpoints every `every` steps."""
 
     @staticmethod
-    def forward(ctx, c, theta, y, ws, scales):
-        ctx.ws, ctx.scales = ws, scales
-        ctx.save_for_backward(c, theta, y)
-        cn = _coefficients(ws)
-        r = y - _ns.synthesize(ws, c)
-
-        def f(i, band):
-            z = _band(c, i) + band
-            s = scales[i - 1]
-            _band(cn, i).copy_(z if s == 0 else soft_threshold(z, theta[s - 1]))
-
-        _ns.analyze(f, ws, r)
-        return cn
+    def forward(ctx, theta, y, ws, scales, iters, every):
+        ctx.ws, ctx.scales, ctx.iters, ctx.every = ws, scales, iters, every
+        ctx.save_for_backward(theta, y)
+        ckpt = {0: None}
+        c = None
+        for k in range(iters):
+            if k and k % every == 0:
+                ckpt[k] = c
+            c = _step(ws, scales, c, theta, y)
+        ctx.ckpt = ckpt
+        return c
 
     @staticmethod
-    def backward(ctx, gn):
-        c, theta, y = ctx.saved_tensors
-        ws, scales = ctx.ws, ctx.scales
-        G = _coefficients(ws)
+    def backward(ctx, gc):
+        theta, y = ctx.saved_tensors
+        ws, scales, iters, every = ctx.ws, ctx.scales, ctx.iters, ctx.every
         gtheta = torch.zeros_like(theta)
-        r = y - _ns.synthesize(ws, c)
-
-        def f(i, band):
-            s = scales[i - 1]
-            gi = gn[:, :, :, i - 1]
-            if s == 0:
-                _band(G, i).copy_(gi)
-                return
-            with torch.enable_grad():
-                z = (_band(c, i) + band).detach().requires_grad_(True)
-                th = theta[s - 1].detach().requires_grad_(True)
-                gz, gt = torch.autograd.grad(soft_threshold(z, th), (z, th), gi)
-            _band(G, i).copy_(gz)
-            gtheta[s - 1] += gt
-
-        _ns.analyze(f, ws, r)
-        u = _ns.synthesize(ws, G)
-
-        def h(i, band):
-            _band(G, i).sub_(band)
-
-        _ns.analyze(h, ws, u)
-        return G, gtheta, u, None, None
+        gy = torch.zeros_like(y)
+        for b in reversed(range(0, iters, every)):
+            e = min(b + every, iters)
+            for k in reversed(range(b, e)):
+                # c_k recomputed from the segment's checkpoint c_b.
+                c = ctx.ckpt[b]
+                for _ in range(b, k):
+                    c = _step(ws, scales, c, theta, y)
+                gc, u = _step_backward(ws, scales, c, theta, y, gc, gtheta)
+                gy += u
+                del c
+        ctx.ckpt = None
+        return gtheta, gy, None, None, None, None
 
 
-def ista(ws, y, theta, iters=10):
+def ista(ws, y, theta, iters=10, checkpoint_every=None):
     """x = A^T c_K after `iters` ISTA steps from c_0 = 0 on the noisy volume y,
     with per-scale thresholds theta (a tensor of length J, differentiable).
-    ws: a workspace of a tight NSSurfacelet for y's size and element type."""
+    ws: a workspace of a tight NSSurfacelet for y's size and element type.
+    checkpoint_every: steps per checkpoint segment (default ceil(sqrt(iters));
+    1 keeps every iterate)."""
     if not isinstance(ws, _ns.NSSurfaceletWorkspace):
         raise ArgumentError("ista needs an NSSurfacelet workspace")
     t = ws.transform
@@ -125,7 +168,8 @@ def ista(ws, y, theta, iters=10):
     scales = scale_of_subbands(t, ws.dims)
     y = y.to(ws.dtype)
     theta = theta.to(device=ws.device, dtype=torch.float32)
-    c = _coefficients(ws).zero_()
-    for _ in range(iters):
-        c = _ISTAStep.apply(c, theta, y, ws, scales)
+    every = checkpoint_every if checkpoint_every is not None else max(1, math.ceil(math.sqrt(iters)))
+    if every < 1:
+        raise ArgumentError(f"checkpoint_every must be ≥ 1, got {every}")
+    c = _ISTA.apply(theta, y, ws, scales, iters, every)
     return _ag.synthesize(ws, c)

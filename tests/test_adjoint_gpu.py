@@ -134,3 +134,102 @@ def test_ns_gradient_matches_finite_differences():
         fd = (L(x64 + eps * d) - L(x64 - eps * d)) / (2 * eps)
         gd = float(np.real(np.sum(np.conj(g) * d)))
         assert abs(gd - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.linalg.norm(g)), (k, gd, fd)
+
+
+def _huber_grad(cr, ci, d):
+    """dH/dRe + i dH/dIm of H(|c|) (Huber, threshold d): c min(1, d / |c|)."""
+    r = np.sqrt(cr ** 2 + ci ** 2)
+    f = np.where(r <= d, 1.0, d / np.maximum(r, 1e-300))
+    return cr * f, ci * f
+
+
+def _huber_sum(cr, ci, d):
+    r = np.sqrt(cr ** 2 + ci ** 2)
+    return float(np.sum(np.where(r <= d, 0.5 * r ** 2, d * (r - 0.5 * d))))
+
+
+def test_ns_gradient_128_complex_matches_synthesize_and_finite_differences():
+    # The gradient workload at its stated size: NSSurfacelet(scales=3,
+    # direction_levels=[2,2,3]) on a 128^3 ComplexF32 volume, the ellipsoid
+    # phantom (8 ellipsoids, semi-axes U[8, 40], N(0, 0.05^2)) times
+    # exp(i phi) with phi a random 4th-order polynomial; loss = sum of
+    # Huber(|c|, 0.1) over the directional coefficients.  The rrule gradient
+    # (surfhip_ns_adjoint of the loss's gradient) is checked against
+    #  - the FP64 oracle's synthesize (= A^T for the tight bank,
+    #    src/transforms/ns_surfacelet.jl:21-25) of the oracle's Huber
+    #    gradient, relative L2 1e-4;
+    #  - 16 random-direction central finite differences of the FP64 oracle's
+    #    loss, relative 1e-4.
+    # The oracle work runs subband by subband over threads (numpy's FFTs
+    # release the GIL): c_k = irfft(X M_k), the Huber gradient's synthesis
+    # sum_k rfft(psi(c_k)) M_k, and for each direction the losses at
+    # c_k +/- eps irfft(D M_k).
+    dev = _dev()
+    dims = (128, 128, 128)
+    delta, eps = 0.1, 1e-3
+    print("case ns gradient 128^3 complex", flush=True)
+    x0 = ellipsoid_phantom(dims, seed=33, noise=0.05).astype(np.float64)
+    x = (x0 * np.exp(1j * _phase(dims, 34))).astype(np.complex64)
+    t = N.NSSurfacelet(scales=3, direction_levels=[2, 2, 3])
+    xd = from_numpy(x, dev).requires_grad_(True)
+    ws = N.make_workspace(t, xd.detach())
+    c = AG.analyze(ws, xd)
+    r = torch.abs(c[..., 1:])
+    loss = torch.sum(torch.where(r <= delta, 0.5 * r ** 2, delta * (r - 0.5 * delta)))
+    loss.backward()
+    g = to_numpy(xd.grad).astype(np.complex128)
+    loss_gpu = float(loss.detach())
+    del c, r, loss
+
+    to = O.NSParams(3, (2, 2, 3))
+    tb = O.tables(to, dims)
+    masks = O.subband_masks(to, dims, tb)
+    x64 = x.astype(np.complex128)
+    Xr, Xi = O.rfft3(x64.real), O.rfft3(x64.imag)
+    rng = np.random.default_rng(35)
+    D = []
+    dirs = []
+    for _ in range(16):
+        d = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
+        d /= np.linalg.norm(d)
+        dirs.append(d)
+        D.append((O.rfft3(d.real), O.rfft3(d.imag)))
+
+    def subband(k):
+        M = masks[k]
+        cr, ci = O.irfft3(Xr * M, dims), O.irfft3(Xi * M, dims)
+        pr, pi = _huber_grad(cr, ci, delta)
+        acc = (O.rfft3(pr) * M, O.rfft3(pi) * M)
+        L0 = _huber_sum(cr, ci, delta)
+        Lpm = np.zeros((16, 2))
+        for j, (Dr, Di) in enumerate(D):
+            ar, ai = O.irfft3(Dr * M, dims), O.irfft3(Di * M, dims)
+            Lpm[j, 0] = _huber_sum(cr + eps * ar, ci + eps * ai, delta)
+            Lpm[j, 1] = _huber_sum(cr - eps * ar, ci - eps * ai, delta)
+        return L0, Lpm, acc
+
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    accr = np.zeros_like(Xr)
+    acci = np.zeros_like(Xi)
+    L = 0.0
+    Lpm = np.zeros((16, 2))
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        for L0, lpm, (ar, ai) in ex.map(subband, range(1, len(masks))):
+            L += L0
+            Lpm += lpm
+            accr += ar
+            acci += ai
+    g_ref = O.irfft3(accr, dims) + 1j * O.irfft3(acci, dims)
+    assert abs(loss_gpu - L) <= 1e-5 * L, (loss_gpu, L)
+    err = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"gradient vs oracle synthesize: rel L2 {err:.2e}", flush=True)
+    assert err <= 1e-4, err
+    worst = 0.0
+    for j, d in enumerate(dirs):
+        fd = (Lpm[j, 0] - Lpm[j, 1]) / (2 * eps)
+        gd = float(np.real(np.sum(np.conj(g) * d)))
+        rel = abs(gd - fd) / max(abs(fd), 1e-3 * np.linalg.norm(g))
+        worst = max(worst, rel)
+        assert rel <= 1e-4, (j, gd, fd)
+    print(f"16 central differences: worst relative deviation {worst:.2e}", flush=True)

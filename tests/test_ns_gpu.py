@@ -60,6 +60,10 @@ CASES = [
     ((64, 64, 256), 2, (1, 2), 6, True),
     # composite radices 14, 9, 6 and 12 (28 = 14 x 2, 54 = 9 x 6, 36 = 12 x 3)
     ((28, 54, 36), 2, (1, 2), 6, True),
+    # four scales, direction_levels [2, 2, 3, 3] (73 subbands), the ISTA
+    # workload's transform: factored and general kernels
+    ((64, 64, 64), 4, (2, 2, 3, 3), 12, True),
+    ((72, 80, 64), 4, (2, 2, 3, 3), 12, True),
 ]
 
 
