@@ -520,33 +520,41 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
   const float2 *Ub = slabs + (size_t)(jb.slab < 0 ? 0 : jb.slab) * H + (size_t)ip * H1 * NC;
   const int pbase = OR == AX3 ? ip * n1n2 : ip * N1;
   const int cstr = OR == AX3 ? N1 : n1n2;
+  // The slab plane is the same for every subband of the block (only plane a
+  // changes): each thread loads its share of its row once and keeps it in
+  // registers across the subbands; dead tiles are selected away below.
+  // Row HS, scaled by its wedge-plane row per subband, goes to UW (thread
+  // kc < NC takes column kc) for slot 0, which transforms z = row 0 + i row
+  // HS: the inverse along kc of each row is real (their spectra are
+  // Hermitian in kc), so the real part is row 0's result and the imaginary
+  // part row HS's.
+  float2 u[RC1];
+  float2 uh = make_float2(0.0f, 0.0f);
+  {
+    const int tid = (int)threadIdx.x;
+    const int slot = tid / RC2, j = tid % RC2;
+    const int row = slot < HS ? slot : 0;
+#pragma unroll
+    for (int r = 0; r < RC1; ++r) u[r] = Ub[(unsigned)(row * NC + j + RC2 * r)];
+    static_assert(NTH >= NC, "one thread per column of row HS");
+    if (tid < NC) uh = Ub[(unsigned)(HS * NC + tid)];
+  }
   for (int p = jb.p0; p < jb.p1; ++p) {
     // Index math is redone per subband from an opaque copy of the thread
-    // index, so that nothing but the loop counter is live across the loop.
+    // index, so that little but the loop counter and the slab row is live
+    // across the loop.
     const int tid = opaque((int)threadIdx.x);
     const int slot = tid / RC2, j = tid % RC2;
     const bool onslot = slot < HS;
     const int row = onslot ? slot : 0;
-    // Issue the slab and mask loads first; dead tiles are selected away
-    // below.  Row HS, scaled by its wedge-plane row, goes to UW (thread
-    // kc < NC takes column kc) for slot 0, which transforms z = row 0 + i
-    // row HS: the inverse along kc of each row is real (their spectra are
-    // Hermitian in kc), so the real part is row 0's result and the imaginary
-    // part row HS's.
     const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
-    float2 u[RC1];
     float wa[RC1];
 #pragma unroll
     for (int r = 0; r < RC1; ++r) {
       const unsigned o = (unsigned)(row * NC + j + RC2 * r);
-      u[r] = Ub[o];
       wa[r] = wab[wunit ? 0u : o];
     }
-    static_assert(NTH >= NC, "one thread per column of row HS");
-    if (tid < NC) {
-      const float2 uh = Ub[(unsigned)(HS * NC + tid)];
-      UW[tid] = wunit ? uh : cscale(uh, wab[(unsigned)(HS * NC + tid)]);
-    }
+    if (tid < NC) UW[tid] = wunit ? uh : cscale(uh, wab[(unsigned)(HS * NC + tid)]);
     if (p == jb.p0) {
       RegFFT<N1, 1>::fill_twiddles(tw1, tw1g, tid, NTH);
       RegFFT<NC, 1>::fill_twiddles(twc, twcg, tid, NTH);
