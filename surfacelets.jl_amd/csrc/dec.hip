@@ -211,9 +211,8 @@ template <int B1, int B2> constexpr int blk2d_xch() {
 }
 
 template <int DIR, int B1, int B2>
-__global__ __launch_bounds__((blk2d_threads<B1, B2>())) void k_blk2d(float2 *S, int h0, int h1, int h2, int K, int m2,
-                                                                  const float2 *__restrict__ tw1g,
-                                                                  const float2 *__restrict__ tw2g) {
+__device__ __forceinline__ void blk2d_body(float2 *S, int h0, int h1, int h2, int K, int m2,
+                                           const float2 *__restrict__ tw1g, const float2 *__restrict__ tw2g) {
   using FA = RegFFT<B1, DIR>;
   using FB = RegFFT<B2, DIR>;
   constexpr int NTH = blk2d_threads<B1, B2>(), TS = B1 + 1;
@@ -296,6 +295,29 @@ __global__ __launch_bounds__((blk2d_threads<B1, B2>())) void k_blk2d(float2 *S, 
     const int a = e % B1, b = e / B1;
     S[base + a + b * s2] = tile[b * TS + a];
   }
+}
+
+template <int DIR, int B1, int B2>
+__global__ __launch_bounds__((blk2d_threads<B1, B2>())) void k_blk2d(float2 *S, int h0, int h1, int h2, int K, int m2,
+                                                                  const float2 *__restrict__ tw1g,
+                                                                  const float2 *__restrict__ tw2g) {
+  blk2d_body<DIR, B1, B2>(S, h0, h1, h2, K, m2, tw1g, tw2g);
+}
+
+// The block FFTs of hourglasses 2 and 3 (K = 1, 2) of one scale in one
+// launch when both take the same tile: blockIdx.y selects the hourglass.
+struct Blk2dPair {
+  float2 *S[2];
+  int h[2][3], K[2], m2[2];
+  unsigned nb[2];
+};
+template <int DIR, int B1, int B2>
+__global__ __launch_bounds__((blk2d_threads<B1, B2>())) void k_blk2d_pair(Blk2dPair P,
+                                                                       const float2 *__restrict__ tw1g,
+                                                                       const float2 *__restrict__ tw2g) {
+  const int y = blockIdx.y;
+  if (blockIdx.x >= P.nb[y]) return;  // the whole block: before any barrier
+  blk2d_body<DIR, B1, B2>(P.S[y], P.h[y][0], P.h[y][1], P.h[y][2], P.K[y], P.m2[y], tw1g, tw2g);
 }
 
 // X = complex(x) * scale, x read with element stride `is`.
@@ -525,8 +547,9 @@ __device__ __forceinline__ int64_t block_stride(const BlockMap &M) {
 // lines 2f and 2f + 1 have Hermitian spectra A and B, so the inverse of
 // A + iB is line 2f + i line 2f + 1.
 template <bool GEN>
-__global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, float *__restrict__ out, int64_t os,
-                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+__device__ __forceinline__ void c2r_lines_body(const float2 *__restrict__ S, float *__restrict__ out, int64_t os,
+                                               const BlockMap &M, int G, const FftPlan1D &pl,
+                                               const float2 *__restrict__ twg) {
   extern __shared__ float2 smem[];
   const int nK = M.nK, hK = M.h[M.K], F = (G + 1) / 2;
   float2 *tw = smem;
@@ -589,15 +612,37 @@ __global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, 
   }
 }
 
+template <bool GEN>
+__global__ __launch_bounds__(DT) void k_c2r_lines(const float2 *__restrict__ S, float *__restrict__ out, int64_t os,
+                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg) {
+  c2r_lines_body<GEN>(S, out, os, M, G, pl, twg);
+}
+
+// The line transforms of the three hourglasses of one scale in one launch
+// (blockIdx.y = K) when their lines have one length: S[K] is hourglass K's
+// half spectrum, M[K] its block map, nb[K] its blocks of G lines.
+struct Lines3 {
+  float2 *S[3];
+  BlockMap M[3];
+  unsigned nb[3];
+};
+template <bool GEN>
+__global__ __launch_bounds__(DT) void k_c2r_lines3(Lines3 L, float *__restrict__ out, int64_t os, int G, FftPlan1D pl,
+                                                  const float2 *__restrict__ twg) {
+  const int K = blockIdx.y;
+  if (blockIdx.x >= L.nb[K]) return;  // the whole block: before any barrier
+  c2r_lines_body<GEN>(L.S[K], out, os, L.M[K], G, pl, twg);
+}
+
 // S lines = r2c along K of the subbands' lines (the first nK/2+1 bins).
 // With `dbl` the interior bins 1 .. nK/2 - 1 are doubled: the adjoint of the
 // unnormalised c2r along K.  Lines 2f and 2f + 1 share one complex FFT
 // Z = FFT(line 2f + i line 2f + 1): A[t] = (Z[t] + conj Z[-t]) / 2 and
 // B[t] = (Z[t] - conj Z[-t]) / 2i.
 template <bool GEN>
-__global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, int64_t is, float2 *__restrict__ S,
-                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg,
-                                                 int dbl) {
+__device__ __forceinline__ void r2c_lines_body(const float *__restrict__ in, int64_t is, float2 *__restrict__ S,
+                                               const BlockMap &M, int G, const FftPlan1D &pl,
+                                               const float2 *__restrict__ twg, int dbl) {
   extern __shared__ float2 smem[];
   const int nK = M.nK, hK = M.h[M.K], F = (G + 1) / 2;
   float2 *tw = smem;
@@ -646,6 +691,21 @@ __global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, 
   }
 }
 
+template <bool GEN>
+__global__ __launch_bounds__(DT) void k_r2c_lines(const float *__restrict__ in, int64_t is, float2 *__restrict__ S,
+                                                 BlockMap M, int G, FftPlan1D pl, const float2 *__restrict__ twg,
+                                                 int dbl) {
+  r2c_lines_body<GEN>(in, is, S, M, G, pl, twg, dbl);
+}
+
+template <bool GEN>
+__global__ __launch_bounds__(DT) void k_r2c_lines3(Lines3 L, const float *__restrict__ in, int64_t is, int G,
+                                                  FftPlan1D pl, const float2 *__restrict__ twg, int dbl) {
+  const int K = blockIdx.y;
+  if (blockIdx.x >= L.nb[K]) return;  // the whole block: before any barrier
+  r2c_lines_body<GEN>(in, is, L.S[K], L.M[K], G, pl, twg, dbl);
+}
+
 // All IRC stages of hourglass K at one scale in one pass.  The stages along
 // axis m (level le = 1..L, block size b = h_m >> le) pair elements at
 // distance b, so the 2^L elements i_m = r + a (h_m >> L), a < 2^L, of one
@@ -662,10 +722,10 @@ struct IrcGroup {
 };
 
 template <int L1, int L2>
-__global__ __launch_bounds__(DT) void k_irc_group(const float2 *__restrict__ X, int n1, int n2,
-                                                  const float *__restrict__ V, float scale, float2 *__restrict__ S,
-                                                  int h1, int h2, int h3, int K, int m1, int m2, IrcGroup F,
-                                                  int mode) {
+__device__ __forceinline__ void irc_group_body(const float2 *__restrict__ X, int n1, int n2,
+                                               const float *__restrict__ V, float scale, float2 *__restrict__ S,
+                                               int h1, int h2, int h3, int K, int m1, int m2, const IrcGroup &F,
+                                               int mode) {
   constexpr int T = L1 + L2, NG = 1 << T;
   const int hd[3] = {h1, h2, h3};
   const int B1 = hd[m1] >> L1, B2 = hd[m2] >> L2, nKh = hd[K];
@@ -749,6 +809,30 @@ __global__ __launch_bounds__(DT) void k_irc_group(const float2 *__restrict__ X, 
   }
 }
 
+template <int L1, int L2>
+__global__ __launch_bounds__(DT) void k_irc_group(const float2 *__restrict__ X, int n1, int n2,
+                                                  const float *__restrict__ V, float scale, float2 *__restrict__ S,
+                                                  int h1, int h2, int h3, int K, int m1, int m2, IrcGroup F,
+                                                  int mode) {
+  irc_group_body<L1, L2>(X, n1, n2, V, scale, S, h1, h2, h3, K, m1, m2, F, mode);
+}
+
+// The grouped IRC passes of the three hourglasses of one scale in one launch
+// (blockIdx.y = K) when all three have the same (L1, L2).
+struct IrcGroup3 {
+  const float *V[3];
+  float scale[3];
+  float2 *S[3];
+  int h[3][3], m1[3], m2[3];
+  IrcGroup F[3];
+};
+template <int L1, int L2>
+__global__ __launch_bounds__(DT) void k_irc_group3(const float2 *__restrict__ X, int n1, int n2, IrcGroup3 P, int mode) {
+  const int K = blockIdx.y;
+  irc_group_body<L1, L2>(X, n1, n2, P.V[K], P.scale[K], P.S[K], P.h[K][0], P.h[K][1], P.h[K][2], K, P.m1[K],
+                         P.m2[K], P.F[K], mode);
+}
+
 // X = sum over K of c_K M_K S_K on the half of axis K, plus (mirror) c_K M_K
 // conj(S_K) on its strictly interior mirror, index i -> (n - i) mod n: the
 // sum of one scale (surfacelet_scale.jl:246-257) as one write of X: for each
@@ -769,7 +853,13 @@ __host__ __device__ __forceinline__ int herm_mirror_src(int j, int n, bool onK) 
   return (n - j) % n;
 }
 
-__global__ void k_hermitian_sum(float2 *__restrict__ X, int n1, int n2, int n3, HermSum H, int mirror) {
+// With `merge`, the pyramid merge of this level follows in the same pass:
+// an element inside the box becomes sqrt(1 - L^2) X + L fac low[idx_low]
+// (k_pyr_merge's arithmetic, src/pyramid/fourier_pyramid.jl:122-127); the
+// box is 0..P and n - P..n - 1 along each axis, so its coordinate is found
+// in O(1) and each box element reads one lowpass element.
+__global__ void k_hermitian_sum(float2 *__restrict__ X, int n1, int n2, int n3, HermSum H, int mirror,
+                                const float2 *__restrict__ low, PyrDev P, float fac, int merge) {
   const int n[3] = {n1, n2, n3};
   const int64_t tot = (int64_t)n1 * n2 * n3;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
@@ -798,6 +888,25 @@ __global__ void k_hermitian_sum(float2 *__restrict__ X, int n1, int n2, int n3, 
         acc = make_float2(acc.x + w * v.x, acc.y - w * v.y);
       }
     }
+    if (merge) {
+      const int len[3] = {P.len1, P.len2, P.len3};
+      int bc[3];
+      bool in = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int Pd = (len[d] - 1) / 2;
+        bc[d] = j[d] <= Pd ? j[d] : (j[d] >= n[d] - Pd ? j[d] - n[d] + 2 * Pd + 1 : -1);
+        in = in && bc[d] >= 0;
+      }
+      if (in) {
+        int64_t xi, li;
+        float L, sq;
+        pyr_factors(P, bc[0], bc[1], bc[2], xi, li, L, sq);
+        const float2 w = low[li];
+        const float f = L * fac;
+        acc = make_float2(sq * acc.x + f * w.x, sq * acc.y + f * w.y);
+      }
+    }
     X[e] = acc;
   }
 }
@@ -816,6 +925,45 @@ __global__ void k_index_map(int kind, int n, int nl, int P, int onK, int64_t *__
     } else {
       out[e] = herm_mirror_src(e, n, onK != 0);
     }
+  }
+}
+
+// The whole 3-D FFT of a 16 x 16 x 16 volume in one block of 256 threads
+// (the coarse lowpass of the pyramids): the volume in LDS (row pitch 17, so
+// no pass meets bank conflicts), one 16-point pencil per thread per axis.
+// io 1: real input rio.in[e stride] scale; io 2: the real part, times
+// rio.scale, to rio.out[e stride]; io 0: complex A in and out.
+template <int DIR>
+__global__ __launch_bounds__(256) void k_fft3_16(float2 *__restrict__ A, RealIO rio, int io) {
+  constexpr int P1 = 17, P2 = 17 * 17;
+  __shared__ float2 V[16 * P2];
+  const int t = threadIdx.x;
+  for (int e = t; e < 4096; e += 256) {
+    const int i0 = e & 15, i1 = (e >> 4) & 15, i2 = e >> 8;
+    V[i0 + P1 * i1 + P2 * i2] =
+        io == 1 ? make_float2(rio.in[(int64_t)e * rio.stride] * rio.scale, 0.0f) : A[e];
+  }
+  __syncthreads();
+  const int a = t & 15, b = t >> 4;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int base = ax == 0 ? P1 * a + P2 * b : (ax == 1 ? a + P2 * b : a + P1 * b);
+    const int step = ax == 0 ? 1 : (ax == 1 ? P1 : P2);
+    float2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = V[base + k * step];
+    dft16<DIR>(v);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) V[base + k * step] = v[k];
+    __syncthreads();
+  }
+  for (int e = t; e < 4096; e += 256) {
+    const int i0 = e & 15, i1 = (e >> 4) & 15, i2 = e >> 8;
+    const float2 z = V[i0 + P1 * i1 + P2 * i2];
+    if (io == 2)
+      rio.out[(int64_t)e * rio.stride] = z.x * rio.scale;
+    else
+      A[e] = z;
   }
 }
 
@@ -994,13 +1142,35 @@ struct surfhip_s_plan {
       SH_HIP(hipStreamWaitEvent(stream, kjoin[K], 0));
     }
   }
+  // Later work of `stream` follows everything already on lane L (1..3;
+  // nothing for lane 0, `stream` itself).
+  void join_lane(int L) {
+    if (L == 0) return;
+    if (rec) {
+      std::vector<hipGraphNode_t> &d0 = rec->deps[0];
+      d0.insert(d0.end(), rec->deps[L].begin(), rec->deps[L].end());
+      std::sort(d0.begin(), d0.end());
+      d0.erase(std::unique(d0.begin(), d0.end()), d0.end());
+      return;
+    }
+    SH_HIP(hipEventRecord(kjoin[L - 1], kst[L - 1]));
+    SH_HIP(hipStreamWaitEvent(stream, kjoin[L - 1], 0));
+  }
+  // The lane of scale j's merged hourglass passes: one lane per scale (of
+  // three).  Any branch makes a recorded graph's replay cost the host more
+  // (at 128^3 per pair: about 0.08 ms with no branch, 0.25 ms with one,
+  // 0.3 ms with one per scale), but the overlap pays: the GPU then needs
+  // about 0.40, 0.35 and 0.31 ms.
+  static int scale_lane(int j) { return 1 + j % 3; }
   std::vector<std::unique_ptr<DecLevelDev>> lev;
   std::vector<std::unique_ptr<DecScaleDev>> sc;
   DevBuf<float> masks;
   DevBuf<float2> filt, tw;
   DevBuf<float2> filt_t;  // transposed planes of the stages along axis 0 of hourglasses 2 and 3
   std::vector<DevBuf<float2>> spectra;  // [j] the spectrum entering level j; [J] the coarse lowpass
-  DevBuf<float2> Sbuf[3];
+  // Half spectra of the three hourglasses, one set per scale, so that the
+  // scales' hourglass passes may overlap.
+  std::vector<std::array<DevBuf<float2>, 3>> Sbuf;
   std::vector<std::pair<int, FftPlan1D>> plans;
 
   const FftPlan1D &plan_for(int n) const {
@@ -1077,47 +1247,117 @@ struct surfhip_s_plan {
   // The block FFTs of hourglass K on S (dims hd, block sizes bsize) along
   // the two axes other than K: one tile pass when K != 0 and the tile fits,
   // else one segmented pass per axis.
+  // Tile sizes (B1 along axis 0, B2 along the other block axis m2) of the
+  // one-pass 2-D block FFT of hourglass K, or false when K takes one
+  // segmented pass per axis (K = 0: axis 0 is its own axis; tiles that are
+  // not register-FFT lengths or exceed 4096 elements).
+  static bool blk_tile(const Dims3 &bsize, int K, int &B1, int &B2, int &m2) {
+    m2 = K == 1 ? 2 : 1;
+    B1 = bsize[0];
+    B2 = bsize[m2];
+    return K != 0 && reg_len(B1) && reg_len(B2) && B1 * B2 <= 4096;
+  }
+// Dispatch a tile launch LAUNCH(D, A, B) on the runtime tile B1 x B2.
+#define SH_BLK_B(LAUNCH, D, A)          \
+  switch (B2) {                         \
+    case 16: LAUNCH(D, A, 16); break;   \
+    case 32: LAUNCH(D, A, 32); break;   \
+    case 64: LAUNCH(D, A, 64); break;   \
+    default: LAUNCH(D, A, 128); break;  \
+  }
+#define SH_BLK_A(LAUNCH, D)                                    \
+  switch (B1) {                                                \
+    case 16: SH_BLK_B(LAUNCH, D, 16) break;                    \
+    case 32: SH_BLK_B(LAUNCH, D, 32) break;                    \
+    case 64:                                                   \
+      switch (B2) {                                            \
+        case 16: LAUNCH(D, 64, 16); break;                     \
+        case 32: LAUNCH(D, 64, 32); break;                     \
+        default: LAUNCH(D, 64, 64); break;                     \
+      }                                                        \
+      break;                                                   \
+    default:                                                   \
+      if (B2 == 16) LAUNCH(D, 128, 16); else LAUNCH(D, 128, 32); \
+      break;                                                   \
+  }
+  // The block FFTs of hourglass K on S (dims hd, block sizes bsize) along
+  // the two axes other than K: one tile pass when K != 0 and the tile fits,
+  // else one segmented pass per axis.
   void block_ffts(int dir, float2 *S, Dims3 hd, Dims3 bsize, int K) {
-    const int m2 = K == 1 ? 2 : 1, B1 = bsize[0], B2 = bsize[m2];
-    if (K != 0 && reg_len(B1) && reg_len(B2) && B1 * B2 <= 4096) {
+    int B1, B2, m2;
+    if (blk_tile(bsize, K, B1, B2, m2)) {
       const int64_t nb = (int64_t)(hd[0] / B1) * (hd[m2] / B2) * hd[K];
       const float2 *t1 = tw.p + plan_for(B1).tw, *t2 = tw.p + plan_for(B2).tw;
 #define SH_BLK(D, A, B) go((k_blk2d<D, A, B>), dim3((unsigned)nb), dim3(blk2d_threads<A, B>()), 0, S, hd[0], hd[1], hd[2], K, m2, t1, t2)
-#define SH_BLK_B(D, A)                  \
-  switch (B2) {                         \
-    case 16: SH_BLK(D, A, 16); break;   \
-    case 32: SH_BLK(D, A, 32); break;   \
-    case 64: SH_BLK(D, A, 64); break;   \
-    default: SH_BLK(D, A, 128); break;  \
-  }
-#define SH_BLK_A(D)                                 \
-  switch (B1) {                                     \
-    case 16: SH_BLK_B(D, 16) break;                 \
-    case 32: SH_BLK_B(D, 32) break;                 \
-    case 64:                                        \
-      switch (B2) {                                 \
-        case 16: SH_BLK(D, 64, 16); break;          \
-        case 32: SH_BLK(D, 64, 32); break;          \
-        default: SH_BLK(D, 64, 64); break;          \
-      }                                             \
-      break;                                        \
-    default:                                        \
-      if (B2 == 16) SH_BLK(D, 128, 16); else SH_BLK(D, 128, 32); \
-      break;                                        \
-  }
-      if (dir < 0) { SH_BLK_A(-1) } else { SH_BLK_A(1) }
-#undef SH_BLK_A
-#undef SH_BLK_B
+      if (dir < 0) { SH_BLK_A(SH_BLK, -1) } else { SH_BLK_A(SH_BLK, 1) }
 #undef SH_BLK
       return;
     }
     for (int d = 0; d < 3; ++d)
       if (d != K) launch_c2c(d, dir, S, hd, bsize[d]);
   }
+
+  // ---- One launch per stage for the three hourglasses of a scale.
+  // A scale qualifies when its spectrum is a cube (the three hourglasses
+  // then have lines of one length and equally many IRC groups), every
+  // hourglass runs its IRC stages as one grouped pass of the same (L1, L2),
+  // and hourglasses 2 and 3 take the same 2-D block tile.  Hourglass 1's
+  // block FFTs stay two segmented passes.
+  bool merged_scale(const DecScaleT &s) const {
+    if (!(s.n[0] == s.n[1] && s.n[1] == s.n[2])) return false;
+    int L[3][2];
+    for (int K = 0; K < 3; ++K) {
+      if (!grouped(s, K)) return false;
+      int m1, m2;
+      transverse(K, m1, m2);
+      L[K][0] = s.lv[K][m1];
+      L[K][1] = s.lv[K][m2];
+      if ((int)s.stages[K].size() != L[K][0] + L[K][1]) return false;
+    }
+    if (L[1][0] != L[0][0] || L[1][1] != L[0][1] || L[2][0] != L[0][0] || L[2][1] != L[0][1]) return false;
+    int B[3][3];
+    for (int K = 1; K < 3; ++K) {
+      Dims3 c, bs;
+      scale_blocks(s, K, c, bs);
+      if (!blk_tile(bs, K, B[K][0], B[K][1], B[K][2])) return false;
+    }
+    return B[1][0] == B[2][0] && B[1][1] == B[2][1];
+  }
+  void block_ffts3(int dir, int j) {
+    const DecScaleT &s = T.scales[j];
+    Dims3 c, bs;
+    scale_blocks(s, 0, c, bs);
+    block_ffts(dir, Sbuf[j][0].p, half_dims(s.n, 0), bs, 0);
+    Blk2dPair P{};
+    int B1 = 0, B2 = 0;
+    for (int y = 0; y < 2; ++y) {
+      const int K = y + 1;
+      int m2;
+      scale_blocks(s, K, c, bs);
+      blk_tile(bs, K, B1, B2, m2);
+      const Dims3 hd = half_dims(s.n, K);
+      P.S[y] = Sbuf[j][K].p;
+      for (int d = 0; d < 3; ++d) P.h[y][d] = hd[d];
+      P.K[y] = K;
+      P.m2[y] = m2;
+      P.nb[y] = (unsigned)((int64_t)(hd[0] / B1) * (hd[m2] / B2) * hd[K]);
+    }
+    const unsigned nb = std::max(P.nb[0], P.nb[1]);
+    const float2 *t1 = tw.p + plan_for(B1).tw, *t2 = tw.p + plan_for(B2).tw;
+#define SH_BLKP(D, A, B) go((k_blk2d_pair<D, A, B>), dim3(nb, 2), dim3(blk2d_threads<A, B>()), 0, P, t1, t2)
+    if (dir < 0) { SH_BLK_A(SH_BLKP, -1) } else { SH_BLK_A(SH_BLKP, 1) }
+#undef SH_BLKP
+  }
+#undef SH_BLK_A
+#undef SH_BLK_B
   // A = FFT_3(rin[i is] scale), the conversion folded into the first pass
   // when every axis has a register-FFT length.
   void fft3_from_real(float2 *A, Dims3 m, const float *rin, int64_t is, float scale) {
     const int64_t n = (int64_t)m[0] * m[1] * m[2];
+    if (m[0] == 16 && m[1] == 16 && m[2] == 16) {
+      go(k_fft3_16<-1>, dim3(1), dim3(256), 0, A, RealIO{rin, nullptr, is, scale}, 1);
+      return;
+    }
     if (!(reg_len(m[0]) && reg_len(m[1]) && reg_len(m[2]))) {
       go(k_to_complex, dim3(grid_for(n)), dim3(DT), 0, rin, is, A, n, scale);
       fft3(-1, A, m);
@@ -1131,6 +1371,10 @@ struct surfhip_s_plan {
   // conversion folded into the last pass when it can be.
   void ifft3_to_real(float2 *A, Dims3 m, float *rout, int64_t os, float scale) {
     const int64_t n = (int64_t)m[0] * m[1] * m[2];
+    if (m[0] == 16 && m[1] == 16 && m[2] == 16) {
+      go(k_fft3_16<1>, dim3(1), dim3(256), 0, A, RealIO{nullptr, rout, os, scale}, 2);
+      return;
+    }
     if (!(reg_len(m[0]) && reg_len(m[1]) && reg_len(m[2]))) {
       fft3(1, A, m);
       go(k_real_out, dim3(grid_for(n)), dim3(DT), 0, A, rout, os, scale, n);
@@ -1151,17 +1395,17 @@ struct surfhip_s_plan {
     transverse(K, m1, m2);
     return s.lv[K][m1] + s.lv[K][m2] <= 4;
   }
-  // mode 0: S = IRC_analysis(X[half_K] V scale); 1: S = IRC_synthesis(S);
-  // 2: S = IRC_analysis^H(S).
-  void irc_group(const DecScaleT &s, int K, int mode, const float2 *X, const float *V, float scale, float2 *S) {
+  // The filter planes of hourglass K's stages for a grouped pass (mode 1
+  // reads the synthesis pair, modes 0 and 2 the analysis pair), with their
+  // strides: the transposed copy where the device keeps one.
+  IrcGroup irc_tables(const DecScaleT &s, int K, int mode) const {
     int m1, m2;
     transverse(K, m1, m2);
-    const int L1 = s.lv[K][m1], L2 = s.lv[K][m2];
     const Dims3 hd = half_dims(s.n, K);
     IrcGroup F{};
     const auto &stg = s.stages[K];
-    if ((int)stg.size() != L1 + L2) fail(SURFHIP_ERR_DEVICE, "internal: IRC stage count");
-    for (int q = 0; q < L1 + L2; ++q) {
+    if ((int)stg.size() != s.lv[K][m1] + s.lv[K][m2]) fail(SURFHIP_ERR_DEVICE, "internal: IRC stage count");
+    for (int q = 0; q < (int)stg.size(); ++q) {
       const int a = mode == 1 ? 2 : 0;
       const bool tr = stg[q].offT[a] >= 0;
       const float2 *base = tr ? filt_t.p : filt.p;
@@ -1171,6 +1415,16 @@ struct surfhip_s_plan {
       F.sc[q] = tr ? 1 : hd[K];
       F.sk[q] = tr ? hd[stg[q].m] : 1;
     }
+    return F;
+  }
+  // mode 0: S = IRC_analysis(X[half_K] V scale); 1: S = IRC_synthesis(S);
+  // 2: S = IRC_analysis^H(S).
+  void irc_group(const DecScaleT &s, int K, int mode, const float2 *X, const float *V, float scale, float2 *S) {
+    int m1, m2;
+    transverse(K, m1, m2);
+    const int L1 = s.lv[K][m1], L2 = s.lv[K][m2];
+    const Dims3 hd = half_dims(s.n, K);
+    const IrcGroup F = irc_tables(s, K, mode);
     const int64_t ng = (int64_t)hd[K] * (hd[m1] >> L1) * (hd[m2] >> L2);
     const dim3 grid(grid_for(ng)), block(DT);
 #define SH_IRCG(A, B)                                                                                                \
@@ -1198,16 +1452,19 @@ struct surfhip_s_plan {
   }
   // X of scale j = the hourglass spectra summed back (with their Hermitian
   // mirrors when `mirror`), each scaled by c[K] and its ritf mask.
-  void hermitian_sum(int j, float2 *X, const float c[3], int mirror) {
+  // With merge_fac > 0 the pyramid merge of level j (low = the coarser
+  // level's spectrum, factor merge_fac) is folded into the same pass.
+  void hermitian_sum(int j, float2 *X, const float c[3], int mirror, float merge_fac = 0.0f) {
     const DecScaleT &s = T.scales[j];
     HermSum H{};
     for (int K = 0; K < 3; ++K) {
-      H.S[K] = Sbuf[K].p;
+      H.S[K] = Sbuf[j][K].p;
       H.M[K] = masks.p + s.mask_off[K];
       H.c[K] = c[K];
     }
     const int64_t NX = (int64_t)s.n[0] * s.n[1] * s.n[2];
-    go(k_hermitian_sum, dim3(grid_for(NX)), dim3(DT), 0, X, s.n[0], s.n[1], s.n[2], H, mirror);
+    go(k_hermitian_sum, dim3(grid_for(NX)), dim3(DT), 0, X, s.n[0], s.n[1], s.n[2], H, mirror,
+       (const float2 *)spectra[j + 1].p, lev[j]->pd, merge_fac, merge_fac > 0.0f ? 1 : 0);
   }
   BlockMap block_map(int j, int K) const {
     const DecScaleT &s = T.scales[j];
@@ -1246,6 +1503,77 @@ struct surfhip_s_plan {
          G, pl, tw.p, dbl);
   }
 
+  void irc_group3(int j, int mode, const float2 *X, const float *scale3) {
+    const DecScaleT &s = T.scales[j];
+    IrcGroup3 P{};
+    int L1 = 0, L2 = 0;
+    int64_t ng = 0;
+    for (int K = 0; K < 3; ++K) {
+      int m1, m2;
+      transverse(K, m1, m2);
+      L1 = s.lv[K][m1];
+      L2 = s.lv[K][m2];
+      const Dims3 hd = half_dims(s.n, K);
+      P.V[K] = masks.p + s.mask_off[K];
+      P.scale[K] = scale3 ? scale3[K] : 1.0f;
+      P.S[K] = Sbuf[j][K].p;
+      for (int d = 0; d < 3; ++d) P.h[K][d] = hd[d];
+      P.m1[K] = m1;
+      P.m2[K] = m2;
+      P.F[K] = irc_tables(s, K, mode);
+      ng = std::max(ng, (int64_t)hd[K] * (hd[m1] >> L1) * (hd[m2] >> L2));
+    }
+    const dim3 grid(grid_for(ng), 3), block(DT);
+#define SH_IRCG3(A, B) go((k_irc_group3<A, B>), grid, block, 0, X, s.n[0], s.n[1], P, mode)
+    switch (L1 * 5 + L2) {
+      case 0: SH_IRCG3(0, 0); break;
+      case 1: SH_IRCG3(0, 1); break;
+      case 2: SH_IRCG3(0, 2); break;
+      case 3: SH_IRCG3(0, 3); break;
+      case 4: SH_IRCG3(0, 4); break;
+      case 5: SH_IRCG3(1, 0); break;
+      case 6: SH_IRCG3(1, 1); break;
+      case 7: SH_IRCG3(1, 2); break;
+      case 8: SH_IRCG3(1, 3); break;
+      case 10: SH_IRCG3(2, 0); break;
+      case 11: SH_IRCG3(2, 1); break;
+      case 12: SH_IRCG3(2, 2); break;
+      case 15: SH_IRCG3(3, 0); break;
+      case 16: SH_IRCG3(3, 1); break;
+      case 20: SH_IRCG3(4, 0); break;
+      default: fail(SURFHIP_ERR_DEVICE, "internal: IRC group size");
+    }
+#undef SH_IRCG3
+  }
+  void lines3(bool c2r, int j, float *coef, int64_t stride, int dbl = 0) {
+    Lines3 L{};
+    int G = 1;
+    size_t lds = 0;
+    unsigned nbmax = 0;
+    int nK = 0;
+    for (int K = 0; K < 3; ++K) {
+      BlockMap M = block_map(j, K);
+      nK = M.nK;
+      const int hK = M.h[K];
+      G = std::max(1, std::min(LINES_MAX, 2048 / nK));
+      const int F = (G + 1) / 2;
+      lds = sizeof(float2) * ((size_t)nK + (size_t)hK * G + 2 * (size_t)nK * F);
+      if (lds > 65536) fail(SURFHIP_ERR_ARG, fmt("axis length %d is too long for the decimated path", nK));
+      const int64_t nlines = (int64_t)M.h[0] * M.h[1] * M.h[2] / hK;
+      L.S[K] = Sbuf[j][K].p;
+      L.M[K] = M;
+      L.nb[K] = (unsigned)((nlines + G - 1) / G);
+      nbmax = std::max(nbmax, L.nb[K]);
+    }
+    const FftPlan1D &pl = plan_for(nK);
+    const bool gen = fft_plan_generic(pl);
+    if (c2r)
+      go(gen ? k_c2r_lines3<true> : k_c2r_lines3<false>, dim3(nbmax, 3), dim3(DT), lds, L, coef, stride, G, pl, tw.p);
+    else
+      go(gen ? k_r2c_lines3<true> : k_r2c_lines3<false>, dim3(nbmax, 3), dim3(DT), lds, L, (const float *)coef, stride,
+         G, pl, tw.p, dbl);
+  }
+
   void analyze_channel(const float *x, float *coef) {
     use_lane(0);
     const int64_t os = nch;
@@ -1259,13 +1587,30 @@ struct surfhip_s_plan {
       go(k_pyr_split, dim3(grid_for((int64_t)pl.nlow[0] * pl.nlow[1] * pl.nlow[2])), dim3(DT), 0,
                          spectra[j].p, low, pd);
       const DecScaleT &s = T.scales[j];
+      if (merged_scale(s)) {
+        // The scale's hourglass passes after the split, the finest scale on
+        // its own lane, so that the next split (of the lowpass) and the
+        // coarser scales do not wait for it.
+        if (scale_lane(j) != 0) fork_k();
+        use_lane(scale_lane(j));
+        float sc3[3];
+        for (int K = 0; K < 3; ++K) {
+          const Dims3 sb = subband_size(s, K);
+          sc3[K] = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
+        }
+        irc_group3(j, 0, spectra[j].p, sc3);
+        block_ffts3(1, j);
+        lines3(true, j, coef, os);
+        use_lane(0);
+        continue;
+      }
       fork_k();
       for (int K = 0; K < 3; ++K) {
         use_lane(K + 1);
         Dims3 hd = half_dims(s.n, K);
         Dims3 sb = subband_size(s, K);
         const float scale = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
-        float2 *S = Sbuf[K].p;
+        float2 *S = Sbuf[j][K].p;
         const int64_t HN = (int64_t)hd[0] * hd[1] * hd[2];
         if (grouped(s, K)) {
           irc_group(s, K, 0, spectra[j].p, masks.p + s.mask_off[K], scale, S);
@@ -1296,15 +1641,29 @@ struct surfhip_s_plan {
     const int64_t is = nch;
     const Dims3 nl = T.pyr.back().nlow;
     float2 *C = spectra[T.J].p;
+    // The merged scales' hourglass passes read only the coefficients: all of
+    // them start at once, each scale on its own lane.
+    fork_k();
+    for (int j = T.J - 1; j >= 0; --j) {
+      if (!merged_scale(T.scales[j])) continue;
+      use_lane(scale_lane(j));
+      lines3(false, j, const_cast<float *>(coef), is);
+      block_ffts3(-1, j);
+      irc_group3(j, 1, nullptr, nullptr);
+    }
+    use_lane(0);
     fft3_from_real(C, nl, coef + T.offsets[0] * nch, is, 1.0f);
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
+      if (merged_scale(s)) {
+        join_lane(scale_lane(j));
+      } else {
       fork_k();
       for (int K = 0; K < 3; ++K) {
         use_lane(K + 1);
         Dims3 hd = half_dims(s.n, K);
-        float2 *S = Sbuf[K].p;
+        float2 *S = Sbuf[j][K].p;
         lines(false, j, K, S, const_cast<float *>(coef), is);
         Dims3 counts, bsize;
         scale_blocks(s, K, counts, bsize);
@@ -1322,13 +1681,11 @@ struct surfhip_s_plan {
       }
       use_lane(0);
       join_k();
+      }
       {
         const float c[3] = {(float)(1.0 / 3.0), (float)(1.0 / 3.0), (float)(1.0 / 3.0)};
-        hermitian_sum(j, X, c, 1);
+        hermitian_sum(j, X, c, 1, lev[j]->pd.D15);
       }
-      const PyrDev &pd = lev[j]->pd;
-      go(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, X,
-                         spectra[j + 1].p, pd, pd.D15);
     }
     const Dims3 n = T.dims;
     const int64_t N = (int64_t)n[0] * n[1] * n[2];
@@ -1343,15 +1700,27 @@ struct surfhip_s_plan {
     const Dims3 nl = T.pyr.back().nlow;
     const int64_t NL = (int64_t)nl[0] * nl[1] * nl[2];
     float2 *C = spectra[T.J].p;
+    fork_k();
+    for (int j = T.J - 1; j >= 0; --j) {
+      if (!merged_scale(T.scales[j])) continue;
+      use_lane(scale_lane(j));
+      lines3(false, j, const_cast<float *>(coef), is, 1);
+      block_ffts3(-1, j);
+      irc_group3(j, 2, nullptr, nullptr);
+    }
+    use_lane(0);
     fft3_from_real(C, nl, coef + T.offsets[0] * nch, is, (float)(1.0 / (double)NL));
     for (int j = T.J - 1; j >= 0; --j) {
       const DecScaleT &s = T.scales[j];
       float2 *X = spectra[j].p;
+      if (merged_scale(s)) {
+        join_lane(scale_lane(j));
+      } else {
       fork_k();
       for (int K = 0; K < 3; ++K) {
         use_lane(K + 1);
         Dims3 hd = half_dims(s.n, K);
-        float2 *S = Sbuf[K].p;
+        float2 *S = Sbuf[j][K].p;
         lines(false, j, K, S, const_cast<float *>(coef), is, 1);
         Dims3 counts, bsize;
         scale_blocks(s, K, counts, bsize);
@@ -1369,17 +1738,15 @@ struct surfhip_s_plan {
       }
       use_lane(0);
       join_k();
+      }
       {
         float c[3];
         for (int K = 0; K < 3; ++K) {
           const Dims3 sb = subband_size(s, K);
           c[K] = (float)(1.0 / ((double)sb[0] * sb[1] * sb[2]));
         }
-        hermitian_sum(j, X, c, 0);
+        hermitian_sum(j, X, c, 0, 1.0f / lev[j]->pd.D15);
       }
-      const PyrDev &pd = lev[j]->pd;
-      go(k_pyr_merge, dim3(grid_for((int64_t)pd.len1 * pd.len2 * pd.len3)), dim3(DT), 0, X,
-                         spectra[j + 1].p, pd, 1.0f / pd.D15);
     }
     const Dims3 n = T.dims;
     ifft3_to_real(spectra[0].p, n, x, (int64_t)nch, 1.0f);
@@ -1518,7 +1885,6 @@ int surfhip_s_plan_create(surfhip_s_plan **out, const surfhip_s_params *p, int32
       P->lev.push_back(std::move(L));
     }
     // Block offsets per scale and hourglass.
-    size_t smax[3] = {0, 0, 0};
     for (size_t j = 0; j < T.scales.size(); ++j) {
       const DecScaleT &s = T.scales[j];
       auto D = std::make_unique<DecScaleDev>();
@@ -1532,12 +1898,15 @@ int surfhip_s_plan_create(surfhip_s_plan **out, const surfhip_s_params *p, int32
         D->first[K] = sub;
         D->boff[K].upload(off);
         sub += nb;
-        Dims3 hd = half_dims(s.n, K);
-        smax[K] = std::max(smax[K], (size_t)hd[0] * hd[1] * hd[2]);
       }
       P->sc.push_back(std::move(D));
     }
-    for (int K = 0; K < 3; ++K) P->Sbuf[K].alloc(smax[K]);
+    P->Sbuf.resize(T.J);
+    for (int j = 0; j < T.J; ++j)
+      for (int K = 0; K < 3; ++K) {
+        const Dims3 hd = half_dims(T.scales[j].n, K);
+        P->Sbuf[j][K].alloc((size_t)hd[0] * hd[1] * hd[2]);
+      }
     P->spectra.resize(T.J + 1);
     for (int j = 0; j <= T.J; ++j) {
       Dims3 n = j < T.J ? T.pyr[j].n : T.pyr.back().nlow;
