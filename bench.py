@@ -88,6 +88,8 @@ def parse():
                          "(scales=3, direction_levels=[2,2,3], pyramid_ratio=2) on the same volume; batch: 64 "
                          "volumes of uneven sizes, the batch against one volume per call")
     ap.add_argument("--lanes", type=int, default=4, help="batch workload: streams of the batch")
+    ap.add_argument("--batch-mode", choices=("both", "plain", "batch"), default="both",
+                    help="batch workload: time both forms (default), or only one of them (for tracing)")
     return ap.parse_args()
 
 
@@ -272,7 +274,8 @@ def bench_batch(args):
         b.run(xs, bc, by)
 
     out = {}
-    for name, fn in (("plain", plain), ("batch", batched)):
+    forms = [f for f in (("plain", plain), ("batch", batched)) if args.batch_mode in ("both", f[0])]
+    for name, fn in forms:
         for _ in range(max(2, args.warmup)):
             fn()
         torch.cuda.synchronize()
@@ -284,6 +287,9 @@ def bench_batch(args):
         torch.cuda.synchronize()
         per = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
         out[name] = step_stats(per)
+    if args.batch_mode != "both":
+        print(json.dumps({"batch_mode": args.batch_mode, "step_ms": out[args.batch_mode]}), flush=True)
+        return
     same = all(torch.equal(bc[k], cs[k]) and torch.equal(by[k], ys[k]) for k in range(64))
     voxsub = sum(int(np.prod(s)) * NS.nsubbands(t, s) for s in shapes)
     ms = out["batch"]["median_ms"]

@@ -37,16 +37,19 @@ template <int DIR> __device__ __forceinline__ void dft16(float2 *x) {
   }
   const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, c2 = 0.70710678118654752440f;
   // w16^m, m = n2 k1 in {1, 2, 3, 2, 4, 6, 3, 6, 9}; forward exp(-2 pi i m / 16).
+  // m = 4 is a quarter turn and m = 2, 6 have |c| = |s|: no multiplications
+  // by 0 or 1 and one product per component.
   auto tw = [&](float2 v, int m) -> float2 {
+    if (m == 4) return rot90<DIR>(v);
+    if (m == 2)
+      return DIR < 0 ? make_float2(c2 * (v.x + v.y), c2 * (v.y - v.x)) : make_float2(c2 * (v.x - v.y), c2 * (v.y + v.x));
+    if (m == 6)
+      return DIR < 0 ? make_float2(c2 * (v.y - v.x), -c2 * (v.x + v.y)) : make_float2(-c2 * (v.x + v.y), c2 * (v.x - v.y));
     float c, s;
     switch (m) {
       case 1: c = c1; s = s1; break;
-      case 2: c = c2; s = c2; break;
       case 3: c = s1; s = c1; break;
-      case 4: c = 0.0f; s = 1.0f; break;
-      case 6: c = -c2; s = c2; break;
-      case 9: c = -c1; s = -s1; break;
-      default: c = 1.0f; s = 0.0f; break;
+      default: c = -c1; s = -s1; break;  // m = 9
     }
     // multiply by (c - i s) forward, (c + i s) inverse
     return DIR < 0 ? make_float2(v.x * c + v.y * s, v.y * c - v.x * s)

@@ -41,8 +41,15 @@ template <int N> constexpr int pencil_stride() { return N + N / FourStep<N>::R1 
 
 // Plane blocks: n1/2 row slots of the tile axis (slot 0 carries rows 0 and
 // n1/2), the pair-contiguous columns of the axis-1 transform, the real output.
+// A row or column pair takes N + N/R1 slots (the padded exchange): for
+// N = 128 that is 8 mod 16, so the two rows that 16 lanes of an 8-byte LDS
+// access touch fall in opposite halves of the 32 banks.  The transposed
+// layouts written between the two passes are XOR-swizzled (plane_col_swz,
+// ana_row_swz) instead of padded, so every wide access of the 128 x 128
+// plane kernels is free of bank conflicts.
+template <int N> constexpr int plane_stride() { return N + N / FourStep<N>::R1; }
 template <int N1, int NC> constexpr int plane_slots() {
-  constexpr int a = (N1 / 2) * pencil_stride<NC>(), b = (NC / 2) * pencil_stride<N1>(), c = N1 * NC / 2;
+  constexpr int a = (N1 / 2) * plane_stride<NC>(), b = (NC / 2) * plane_stride<N1>(), c = N1 * NC / 2;
   constexpr int m = a > b ? a : b;
   return m > c ? m : c;
 }
@@ -486,8 +493,16 @@ __global__ __launch_bounds__(pencil_threads<NP>()) void kP_fwd(const float2 *__r
 // column-major (column stride CS) for the transform along k1, whose outputs
 // are stored straight from registers.
 template <int NC> constexpr int col_stride() { return NC >= 64 ? 68 : 36; }
+// Analysis: row k1 < n1/2 of column kc lives at slot k1 ^ ana_row_swz(kc) of
+// the column (row n1/2 unswizzled), so the eight columns that 16 lanes write
+// at once spread over all banks.  kc and kc ^ 1 share the swizzle.
+__device__ __forceinline__ int ana_row_swz(int kc) { return 2 * ((kc >> 2) & 3); }
+// Synthesis: column kc of row k1 lives at slot kc ^ plane_col_swz(k1) of the
+// row, so the eight rows that 16 lanes write at once spread over all banks.
+// The swizzle is even: columns 2 pr and 2 pr + 1 stay adjacent.
+__device__ __forceinline__ int plane_col_swz(int k1) { return 2 * ((k1 >> 1) & 7); }
 template <int N1, int NC> constexpr int ana_slots() {
-  constexpr int a = (N1 / 2) * pencil_stride<NC>(), b = NC * col_stride<N1>(), c = (NC / 2) * pencil_stride<N1>();
+  constexpr int a = (N1 / 2) * plane_stride<NC>(), b = NC * col_stride<N1>(), c = (NC / 2) * plane_stride<N1>();
   constexpr int m = a > b ? a : b;
   return m > c ? m : c;
 }
@@ -503,7 +518,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     const PlaneJob *__restrict__ jobs, float *__restrict__ out, int64_t substride, int os, float scale,
     const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg) {
   constexpr int tpw = tp_of<NC>();  // columns per live-tile bit
-  constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = pencil_stride<NC>(), PP = pencil_stride<N1>();
+  constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = plane_stride<NC>(), PP = plane_stride<N1>();
   constexpr int CS = col_stride<N1>();
   constexpr int NTH = ana_threads<N1, NC>();
   constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
@@ -596,12 +611,13 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
 #pragma unroll
           for (int qq = 0; qq < RC2; ++qq) {
             const float2 z = act ? v[uu * RC2 + qq] : make_float2(0.0f, 0.0f);
-            float2 *lc = L + ((j * RCR + uu) + RC1 * qq) * CS;
+            const int kc = (j * RCR + uu) + RC1 * qq;
+            float2 *lc = L + kc * CS;
             if (slot == 0) {
-              lc[0] = make_float2(z.x, 0.0f);
+              lc[ana_row_swz(kc)] = make_float2(z.x, 0.0f);
               lc[HS] = make_float2(z.y, 0.0f);
             } else {
-              lc[slot] = z;
+              lc[slot ^ ana_row_swz(kc)] = z;
             }
           }
       }
@@ -614,6 +630,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     float2 w[R1];
     if (on) {
       const float2 *ca = L + (2 * pr) * CS, *cb = ca + CS;
+      const int gz = ana_row_swz(2 * pr);
 #pragma unroll
       for (int r = 0; r < R1; ++r) {
         // Row t = jj + R2 r: below HS = R1 R2 / 2 for r < R1 / 2, above it
@@ -622,7 +639,9 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
         const int t = jj + R2 * r;
         const bool mir = 2 * r > R1 || (2 * r == R1 && jj > 0);
         const bool real = (r == 0 || 2 * r == R1) && jj == 0;
-        float2 A = ca[mir ? N1 - t : t], B = cb[mir ? N1 - t : t];
+        const int k1 = mir ? N1 - t : t;
+        const int sk = 2 * r == R1 && jj == 0 ? HS : k1 ^ gz;  // row HS is not swizzled
+        float2 A = ca[sk], B = cb[sk];
         if (mir) {
           A = conjf2(A);
           B = conjf2(B);
@@ -706,7 +725,7 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
     const uint32_t *__restrict__ live, const PlaneJob *__restrict__ jobs, float2 *__restrict__ slabs, int64_t H,
     const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg) {
   constexpr int tpw = tp_of<NC>();  // columns per live-tile bit
-  constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = pencil_stride<NC>(), PP = pencil_stride<N1>();
+  constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = plane_stride<NC>(), PP = plane_stride<N1>();
   constexpr int NTH = plane_threads<N1, NC>();
   constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
   constexpr int C1 = FourStep<N1>::R1, C2 = FourStep<N1>::R2, CR = C1 / C2;
@@ -794,28 +813,28 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
             if (on) {
               const float2 Z = v[u * C2 + q], Zc = conjf2(Zm);
               const float2 d = csub(Z, Zc);
-              float2 *lc = L + (jj * CR + u + C1 * q) * PS + 2 * pr;
-              lc[0] = cscale(cadd(Z, Zc), 0.5f);
-              lc[1] = make_float2(0.5f * d.y, -0.5f * d.x);
+              const int k1 = jj * CR + u + C1 * q;
+              const float2 A = cscale(cadd(Z, Zc), 0.5f);
+              *reinterpret_cast<float4 *>(L + k1 * PS + ((2 * pr) ^ plane_col_swz(k1))) =
+                  make_float4(A.x, A.y, 0.5f * d.y, -0.5f * d.x);
             }
           }
         // Rows 0 and HS are real: slot 0 takes row 0 + i row HS (replacing
         // the row 0 pair written above by the same thread).
-        if (on && jj == 0) {
-          L[2 * pr] = make_float2(v[0].x, v[C2 / 2].x);
-          L[2 * pr + 1] = make_float2(v[0].y, v[C2 / 2].y);
-        }
+        if (on && jj == 0)  // row 0: no swizzle
+          *reinterpret_cast<float4 *>(L + 2 * pr) = make_float4(v[0].x, v[C2 / 2].x, v[0].y, v[C2 / 2].y);
       }
     }
     __syncthreads();
     {
       const int sl = t / RC2, jr = t % RC2;
       float2 *pb = L + (onslot ? sl : 0) * PS;
+      const int cz = plane_col_swz(sl);
       float2 v[RC1];
       // The exchange region of a row is private to its RC2 lanes.
       if (act) {
 #pragma unroll
-        for (int r = 0; r < RC1; ++r) v[r] = pb[jr + RC2 * r];
+        for (int r = 0; r < RC1; ++r) v[r] = pb[(jr + RC2 * r) ^ cz];
         RegFFT<NC, -1>::phase1(v, jr, twc);
       }
       wave_sync();
