@@ -35,9 +35,12 @@ namespace surfhip {
 // Pencils per tile and threads per pencil block (R2 threads per pencil).
 template <int N> constexpr int tp_of() { return 16; }
 template <int N> constexpr int pencil_threads() { return tp_of<N>() * FourStep<N>::R2; }
-// LDS slots of one pencil: the padded exchange needs N + N/R1, and an odd
-// stride keeps accesses of neighbouring pencils in different banks.
-template <int N> constexpr int pencil_stride() { return N + N / FourStep<N>::R1 + 1; }
+// LDS slots of one pencil: the padded exchange needs N + N/R1, and the
+// stride spreads the eight pencils that 16 lanes touch at once over the
+// banks: for N = 128 (8 lanes per pencil) 138 slots = 10 mod 16 makes the
+// loads and both sides of the exchange conflict-free; other lengths take
+// the next odd stride.
+template <int N> constexpr int pencil_stride() { return N == 128 ? 138 : N + N / FourStep<N>::R1 + 1; }
 
 // Plane blocks: n1/2 row slots of the tile axis (slot 0 carries rows 0 and
 // n1/2), the pair-contiguous columns of the axis-1 transform, the real output.
