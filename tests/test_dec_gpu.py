@@ -159,6 +159,8 @@ def test_surfacelet_128_phantom_config():
     assert _rel(to_numpy(c.data), flat_ref) <= RTOL
     xr = to_numpy(S.synthesize(t, c, workspace=ws))
     assert _rel(xr, x) <= RTOL
+    print(f"decimated 128^3 phantom: analysis rel L2 vs oracle {_rel(to_numpy(c.data), flat_ref):.2e}, "
+          f"reconstruction {_rel(xr, x):.2e}", flush=True)
 
 
 @pytest.mark.gpu
