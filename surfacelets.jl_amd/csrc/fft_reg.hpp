@@ -1,12 +1,15 @@
 // Register-resident four-step FFT of one pencil held by R2 threads (gfx950).
 //
-// A length-N pencil (N = R1 R2, R2 | R1) lives in LDS at consecutive slots
+// A length-N pencil (N = R1 R2) lives in LDS at consecutive slots
 // buf[slot(e)], e in [0, N).  Thread j in [0, R2) of the pencil:
 //   phase 1: loads x[j + R2 r] (r < R1), runs an R1-point DFT in registers and
 //            multiplies output s by w_N^(j s);
 //   exchange: stores output s at slot s R2 + j;
-//   phase 2: loads the R1 slots [j R1, (j+1) R1) -- R1/R2 rows of R2 -- and
-//            runs R1/R2 R2-point DFTs: row s gives X[s + R1 q], q < R2.
+//   phase 2: loads rows s = j ROWS + u (u < ROWS = ceil(R1 / R2), s < R1) of
+//            R2 slots each and runs an R2-point DFT per row: row s gives
+//            X[s + R1 q], q < R2.  When R2 does not divide R1 (N = 80, 96,
+//            112: R1 = 10, 12, 14) the last threads hold fewer rows or none
+//            (row_ok); their register rows are computed but carry nothing.
 // X[s + R1 q] = sum_j w_R2^(j q) w_N^(j s) sum_r x[j + R2 r] w_R1^(r s) is the
 // four-step factorisation of the DFT; DIR = -1 forward, +1 inverse
 // (unnormalised).  The caller places __syncthreads() between the phases.
@@ -24,6 +27,9 @@ template <> struct FourStep<32> { static constexpr int R1 = 8, R2 = 4; };
 template <> struct FourStep<64> { static constexpr int R1 = 8, R2 = 8; };
 template <> struct FourStep<128> { static constexpr int R1 = 16, R2 = 8; };
 template <> struct FourStep<256> { static constexpr int R1 = 16, R2 = 16; };
+template <> struct FourStep<80> { static constexpr int R1 = 10, R2 = 8; };
+template <> struct FourStep<96> { static constexpr int R1 = 12, R2 = 8; };
+template <> struct FourStep<112> { static constexpr int R1 = 14, R2 = 8; };
 
 template <int DIR> __device__ __forceinline__ void dft16(float2 *x) {
   // 16 = 4 x 4: DFT-4 over n1 for each n2, twiddle w16^(n2 k1), DFT-4 over n2.
@@ -73,6 +79,7 @@ template <int R, int DIR> __device__ __forceinline__ void dftR(float2 *v) {
   else if constexpr (R == 4) dft4<DIR>(v);
   else if constexpr (R == 8) dft8<DIR>(v);
   else if constexpr (R == 16) dft16<DIR>(v);
+  else dft_radix<R, DIR>(v);  // 10, 12, 14: two-level composites (fft.hpp)
 }
 
 // Twiddle w_N^m from a table of the N forward twiddles (tw[m] = exp(-2 pi i m / N)).
@@ -83,11 +90,22 @@ template <int DIR> __device__ __forceinline__ float2 tmul(float2 v, const float2
 }
 
 template <int N, int DIR> struct RegFFT {
-  static constexpr int R1 = FourStep<N>::R1, R2 = FourStep<N>::R2, ROWS = R1 / R2;
+  static constexpr int R1 = FourStep<N>::R1, R2 = FourStep<N>::R2, ROWS = (R1 + R2 - 1) / R2;
+  // Register array length of a pencil thread: R1 for phase 1, ROWS rows of R2
+  // for phase 2.
+  static constexpr int NV = ROWS * R2 > R1 ? ROWS * R2 : R1;
+  // Whether row u of thread j exists (always when R2 divides R1).
+  __device__ static __forceinline__ bool row_ok(int j, int u) { return R1 % R2 == 0 || j * ROWS + u < R1; }
+  // Exchange slot of element e: one pad slot per R1 slots, so that the R1
+  // consecutive slots each thread reads in phase 2 start in different banks;
+  // when R2 does not divide R1, one pad slot per row of R2 (a shift, where
+  // e / R1 would be a division).  A pencil's exchange spans N + PAD slots.
+  static constexpr int PAD = R1 % R2 == 0 ? N / R1 : N / R2;
+  __device__ static __forceinline__ int xpos(int e) { return R1 % R2 == 0 ? e + e / R1 : e + e / R2; }
   // Phase 1 on values already in registers (v[r] = x[j + R2 r]).  twT is the
   // twiddle table in phase-1 order (fill_twiddles): thread j reads
   // twT[s R2 + j] = w_N^(j s), one base address and constant offsets.
-  __device__ static __forceinline__ void phase1(float2 (&v)[R1], int j, const float2 *twT) {
+  __device__ static __forceinline__ void phase1(float2 (&v)[NV], int j, const float2 *twT) {
     dftR<R1, DIR>(v);
     const float2 *tj = twT + j;
 #pragma unroll
@@ -98,9 +116,31 @@ template <int N, int DIR> struct RegFFT {
   __device__ static __forceinline__ void fill_twiddles(float2 *twT, const float2 *tw, int first, int step) {
     for (int m = first; m < N; m += step) twT[m] = tw[(m % R2) * (m / R2)];
   }
+  // The exchange stores of phase 1: output s of thread j to slot s R2 + j.
+  __device__ static __forceinline__ void store_cols(const float2 (&v)[NV], float2 *pb, int j) {
+#pragma unroll
+    for (int s = 0; s < R1; ++s) pb[xpos(s * R2 + j)] = v[s];
+  }
+  // The exchange loads of phase 2: v[u R2 + q] = slot (j ROWS + u) R2 + q for
+  // the rows thread j holds (the R1 consecutive slots from j R1 when R2
+  // divides R1).
+  __device__ static __forceinline__ void load_rows(float2 (&v)[NV], const float2 *pb, int j) {
+    if constexpr (R1 % R2 == 0) {
+#pragma unroll
+      for (int e = 0; e < R1; ++e) v[e] = pb[xpos(j * R1 + e)];
+    } else {
+#pragma unroll
+      for (int u = 0; u < ROWS; ++u)
+        if (row_ok(j, u)) {
+          const float2 *row = pb + (j * ROWS + u) * (R2 + 1);  // slot s R2 + q sits at s (R2 + 1) + q
+#pragma unroll
+          for (int q = 0; q < R2; ++q) v[u * R2 + q] = row[q];
+        }
+    }
+  }
   // Phase 2 on v[u R2 + q'] = slot((j ROWS + u) R2 + q'); results in place:
   // v[u R2 + q] = X[(j ROWS + u) + R1 q].
-  __device__ static __forceinline__ void phase2(float2 (&v)[R1]) {
+  __device__ static __forceinline__ void phase2(float2 (&v)[NV]) {
 #pragma unroll
     for (int u = 0; u < ROWS; ++u) dftR<R2, DIR>(v + u * R2);
   }

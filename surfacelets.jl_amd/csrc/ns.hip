@@ -3,7 +3,8 @@
 // Every subband is irfft(X . M_k) with X = rfft(x) and M_k the real mask of
 // subband k (src/transforms/ns_surfacelet.jl:8-32); synthesis is
 // irfft(sum_k rfft(c_k) . M_k).  This file holds the general-size path and
-// the plan; the power-of-two grids run the factored kernels of ns_fused.hip.
+// the plan; grids with every length in {32, 64, 80, 96, 112, 128} run the
+// factored kernels of ns_fused.hip.
 // A 3-D real FFT here is a paired-row r2c/c2r along axis 1 (the contiguous
 // axis of the volume) and one LDS-resident 2-D transform of each (k2, k3)
 // plane (k_plane), or where a plane exceeds LDS one pass per axis over
@@ -574,8 +575,9 @@ __global__ void k_zero(float2 *__restrict__ p, int64_t n) {
 using namespace surfhip;
 
 // A plan runs one of two kernel sets on the same tables: the factored path
-// (ns_fused.hip) on the power-of-two grids it covers, or the general path
-// above on any other size.  Both evaluate the same masks.
+// (ns_fused.hip) on the grids it covers (every length in {32, 64, 80, 96,
+// 112, 128}), or the general path above on any other size.  Both evaluate
+// the same masks.
 struct surfhip_ns_plan {
   NSTables T;
   int dtype = SURFHIP_F32;

@@ -1,5 +1,5 @@
-// Factored NSSurfacelet kernels for power-of-two grids with n1, n2, n3 in
-// {32, 64, 128} (the factorisation is described in ns_fused.hpp).
+// Factored NSSurfacelet kernels for grids with n1, n2, n3 in {32, 64, 80, 96,
+// 112, 128} (the factorisation is described in ns_fused.hpp).
 //
 // Pencil passes (one 1-D transform along the group's pencil axis per block
 // of TP pencils, a "tile"): the spectra X / acc are pencil-major
@@ -15,7 +15,8 @@
 // Addresses are formed as block-uniform base pointers plus 32-bit offsets.
 // Every kernel asks for at least 4 waves per SIMD (at most 128 VGPRs): two
 // 8-wave plane blocks or four 4-wave pencil blocks per CU, which is what
-// their LDS allows.
+// their LDS allows; plane blocks of other sizes ask for the fewest waves that
+// keep their block count (plane_waves).
 //
 // Bitwise reproducibility: the streaming entry points run these same
 // kernels with smaller job or unit lists; the arithmetic of a subband never
@@ -40,7 +41,7 @@ template <int N> constexpr int pencil_threads() { return tp_of<N>() * FourStep<N
 // banks: for N = 128 (8 lanes per pencil) 138 slots = 10 mod 16 makes the
 // loads and both sides of the exchange conflict-free; other lengths take
 // the next odd stride.
-template <int N> constexpr int pencil_stride() { return N == 128 ? 138 : N + N / FourStep<N>::R1 + 1; }
+template <int N> constexpr int pencil_stride() { return N == 128 ? 138 : N + RegFFT<N, 1>::PAD + 1; }
 
 // Plane blocks: n1/2 row slots of the tile axis (slot 0 carries rows 0 and
 // n1/2), the pair-contiguous columns of the axis-1 transform, the real output.
@@ -50,7 +51,7 @@ template <int N> constexpr int pencil_stride() { return N == 128 ? 138 : N + N /
 // layouts written between the two passes are XOR-swizzled (plane_col_swz,
 // ana_row_swz) instead of padded, so every wide access of the 128 x 128
 // plane kernels is free of bank conflicts.
-template <int N> constexpr int plane_stride() { return N + N / FourStep<N>::R1; }
+template <int N> constexpr int plane_stride() { return N + RegFFT<N, 1>::PAD; }
 template <int N1, int NC> constexpr int plane_slots() {
   constexpr int a = (N1 / 2) * plane_stride<NC>(), b = (NC / 2) * plane_stride<N1>(), c = N1 * NC / 2;
   constexpr int m = a > b ? a : b;
@@ -61,10 +62,18 @@ template <int N1, int NC> constexpr int plane_threads() {
   constexpr int m = a > b ? a : b;
   return (m + 63) / 64 * 64;
 }
-
-// Exchange slot of element e: one pad slot per R1, so that the R1
-// consecutive slots each thread reads in phase 2 start in different banks.
-template <int R1> __device__ __forceinline__ int xpos(int e) { return e + e / R1; }
+// Waves per SIMD a plane kernel of NTH threads and LDS bytes of shared memory
+// asks for: the fewest that still let a CU hold as many blocks as 4 waves per
+// SIMD would (LDS permitting).  A 6-wave block fits twice either way, so it
+// gets 3 waves per SIMD and 168 registers instead of 128 (no spills); 8-wave
+// blocks keep 4.
+template <int NTH, int LDS> constexpr int plane_waves() {
+  constexpr int W = NTH / 64, lds_blocks = 160 * 1024 / LDS;
+  constexpr int target = 16 / W < lds_blocks ? 16 / W : lds_blocks;
+  for (int m = 1; m < 4; ++m)
+    if ((4 * m / W < lds_blocks ? 4 * m / W : lds_blocks) >= target) return m;
+  return 4;
+}
 
 // Synchronisation of the lanes of one wave around LDS traffic that only they
 // share (the exchange of a pencil whose threads are consecutive lanes): a
@@ -111,8 +120,9 @@ __device__ __forceinline__ float2 fma2(float2 a, float s, float2 c) {
 // thread 0 mirrors its own element (0, -q mod R2)), so one shuffle per
 // component fetches it.  Every lane of the pencil must call it.
 template <int N>
-__device__ __forceinline__ float2 fft_mirror(const float2 (&v)[FourStep<N>::R1], int j, int u, int q) {
-  constexpr int R1 = FourStep<N>::R1, R2 = FourStep<N>::R2, ROWS = R1 / R2;
+__device__ __forceinline__ float2 fft_mirror(const float2 (&v)[RegFFT<N, 1>::NV], int j, int u, int q) {
+  constexpr int R1 = FourStep<N>::R1, R2 = FourStep<N>::R2, ROWS = RegFFT<N, 1>::ROWS;
+  static_assert(R1 % R2 == 0 || (ROWS == 2 && R1 % 2 == 0), "the mirror row keeps the row index u");
   const int s = j * ROWS + u;
   const int src = s == 0 ? 0 : (R1 - s) / ROWS;
   const float2 x = v[u * R2 + (R2 - 1 - q)];
@@ -127,29 +137,27 @@ template <int N, int DIR>
 __device__ __forceinline__ void pencil_fft(float2 *pb, const float2 *tw, int j, bool active) {
   using F = RegFFT<N, DIR>;
   constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS;
-  float2 v[R1];
+  float2 v[F::NV];
   if (active) {
 #pragma unroll
     for (int r = 0; r < R1; ++r) v[r] = pb[j + R2 * r];
     F::phase1(v, j, tw);
   }
   __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
-  }
+  if (active) F::store_cols(v, pb, j);
   __syncthreads();
   if (active) {
-#pragma unroll
-    for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
+    F::load_rows(v, pb, j);
     F::phase2(v);
   }
   __syncthreads();
   if (active) {
 #pragma unroll
     for (int u = 0; u < ROWS; ++u)
+      if (F::row_ok(j, u)) {
 #pragma unroll
-      for (int q = 0; q < R2; ++q) pb[(j * ROWS + u) + R1 * q] = v[u * R2 + q];
+        for (int q = 0; q < R2; ++q) pb[(j * ROWS + u) + R1 * q] = v[u * R2 + q];
+      }
   }
   __syncthreads();
 }
@@ -227,29 +235,26 @@ template <int NP, int OR, int NTH = pblock_threads<NP>()> struct TileElem {
 // exchange needs only wave-level fences.
 template <int NP, int DIR, bool WAVE = false>
 __device__ __forceinline__ void tile_fft(float2 *pb, const float2 *tw, int j, bool active,
-                                         float2 (&v)[FourStep<NP>::R1]) {
-  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2;
+                                         float2 (&v)[RegFFT<NP, 1>::NV]) {
+  using F = RegFFT<NP, DIR>;
+  constexpr int R1 = F::R1, R2 = F::R2;
   if (active) {
 #pragma unroll
     for (int r = 0; r < R1; ++r) v[r] = pb[j + R2 * r];
-    RegFFT<NP, DIR>::phase1(v, j, tw);
+    F::phase1(v, j, tw);
   }
   if (WAVE)
     wave_sync();
   else
     __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
-  }
+  if (active) F::store_cols(v, pb, j);
   if (WAVE)
     wave_sync();
   else
     __syncthreads();
   if (active) {
-#pragma unroll
-    for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
-    RegFFT<NP, DIR>::phase2(v);
+    F::load_rows(v, pb, j);
+    F::phase2(v);
   }
 }
 
@@ -265,7 +270,9 @@ __global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *
                                                                   const int4 *__restrict__ units,
                                                                   const float2 *__restrict__ twg) {
   constexpr int TP = tp_of<NP>(), NTH = pblock_threads<NP>(), HT = half_threads<NP>(), PS = pencil_stride<NP>();
-  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2, ROWS = R1 / R2, E = TP * NP / NTH;
+  using F = RegFFT<NP, 1>;
+  constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS, E = TP * NP / NTH;
+  static_assert(E * NTH == TP * NP, "a tile's elements share out evenly");
   __shared__ float2 buf[2][TP * PS];
   __shared__ float2 tw[NP];
   const int tid = threadIdx.x;
@@ -305,15 +312,17 @@ __global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *
     }
     __syncthreads();
     const bool act = lv[half];
-    float2 v[R1];
+    float2 v[F::NV];
     tile_fft<NP, 1, true>(buf[half] + pq * PS, tw, j, act, v);
     if (act) {
       float2 *dst = slabs + (size_t)(gd.slab0 + q0 + half) * H + (size_t)k1 * nt + c0;
 #pragma unroll
       for (int uu = 0; uu < ROWS; ++uu)
+        if (F::row_ok(j, uu)) {
 #pragma unroll
-        for (int qq = 0; qq < R2; ++qq)
-          dst[(unsigned)(((j * ROWS + uu) + R1 * qq) * rs + pq)] = v[uu * R2 + qq];
+          for (int qq = 0; qq < R2; ++qq)
+            dst[(unsigned)(((j * ROWS + uu) + R1 * qq) * rs + pq)] = v[uu * R2 + qq];
+        }
     }
     __syncthreads();
   }
@@ -340,7 +349,11 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
                                                                const float2 *__restrict__ twg) {
   constexpr int TP = tp_of<NP>(), NQ = syn_quarters<NP>(), NTH = syn_threads<NP>(), QT = half_threads<NP>();
   constexpr int PS = pencil_stride<NP>();
-  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2, ROWS = R1 / R2, E = TP * NP / NTH;
+  using F = RegFFT<NP, -1>;
+  constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS, E = (TP * NP + NTH - 1) / NTH;
+  // Whether element m of this thread's share lies in the tile (the last
+  // element of some threads does not when NTH does not divide TP NP).
+  auto in_tile = [&](int m) { return E * NTH == TP * NP || m < E - 1 || (int)threadIdx.x + NTH * m < TP * NP; };
   using TE = TileElem<NP, OR, NTH>;
   __shared__ float2 buf[NQ][TP * PS];
   __shared__ float2 tw[NP];
@@ -353,7 +366,7 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const TE t(tid, m, c0, n3);
-    a[m] = (OR == AX3 && has_acc) ? acc[kb + t.off] : make_float2(0.0f, 0.0f);
+    a[m] = (OR == AX3 && has_acc && in_tile(m)) ? acc[kb + t.off] : make_float2(0.0f, 0.0f);
   }
   __syncthreads();
   // Thread lt of a quarter: pencil pq = pq_lo + PL pq_hi, lane j; the R2
@@ -375,7 +388,7 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
     }
     if (!any) continue;
     const bool act = lv[quarter];
-    float2 v[R1];
+    float2 v[F::NV];
     if (act) {
       const float2 *src = slabs + (size_t)slist[s0 + quarter].x * H + (size_t)k1 * nt + c0;
 #pragma unroll
@@ -383,22 +396,22 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
     }
     float2 *pb = buf[quarter] + pq * PS;
     if (act) {
-      RegFFT<NP, -1>::phase1(v, j, tw);
-#pragma unroll
-      for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
+      F::phase1(v, j, tw);
+      F::store_cols(v, pb, j);
     }
     wave_sync();  // the exchange region of a pencil is private to its R2 lanes
     if (act) {
-#pragma unroll
-      for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
-      RegFFT<NP, -1>::phase2(v);
+      F::load_rows(v, pb, j);
+      F::phase2(v);
     }
     wave_sync();
     if (act) {
 #pragma unroll
       for (int uu = 0; uu < ROWS; ++uu)
+        if (F::row_ok(j, uu)) {
 #pragma unroll
-        for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
+          for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -410,7 +423,7 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const TE t(tid, m, c0, n3);
-        a[m] = fma2(buf[h][t.pp * PS + t.kp], Gk[t.off] * wb.at(t.k2, t.k3), a[m]);
+        if (in_tile(m)) a[m] = fma2(buf[h][t.pp * PS + t.kp], Gk[t.off] * wb.at(t.k2, t.k3), a[m]);
       }
     }
     __syncthreads();
@@ -419,25 +432,28 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const TE t(tid, m, c0, n3);
-      acc[kb + t.off] = a[m];
+      if (in_tile(m)) acc[kb + t.off] = a[m];
     }
     return;
   }
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const TE t(tid, m, c0, n3);
-    buf[0][t.pp * PS + t.kp] = a[m];
+    if (in_tile(m)) buf[0][t.pp * PS + t.kp] = a[m];
   }
   __syncthreads();
-  float2 v[R1];
+  float2 v[F::NV];
   tile_fft<NP, 1, true>(buf[0] + pq * PS, tw, j, quarter == 0, v);
   if (quarter == 0) {
     float2 *dst = V + (size_t)k1 * n2 + c0;
     const int vs = h1 * n2;
 #pragma unroll
     for (int uu = 0; uu < ROWS; ++uu)
+      if (F::row_ok(j, uu)) {
 #pragma unroll
-      for (int qq = 0; qq < R2; ++qq) dst[(unsigned)(((j * ROWS + uu) + R1 * qq) * vs + pq)] = v[uu * R2 + qq];
+        for (int qq = 0; qq < R2; ++qq)
+          dst[(unsigned)(((j * ROWS + uu) + R1 * qq) * vs + pq)] = v[uu * R2 + qq];
+      }
   }
 }
 
@@ -446,7 +462,9 @@ template <int NP>
 __global__ __launch_bounds__(pencil_threads<NP>()) void kP_fwd(const float2 *__restrict__ T, float2 *__restrict__ X,
                                                                int h1, int n2, const float2 *__restrict__ twg) {
   constexpr int TP = tp_of<NP>(), NTH = pencil_threads<NP>(), PS = pencil_stride<NP>();
-  constexpr int R1 = FourStep<NP>::R1, R2 = FourStep<NP>::R2, ROWS = R1 / R2, E = TP * NP / NTH;
+  using F = RegFFT<NP, -1>;
+  constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS, E = TP * NP / NTH;
+  static_assert(E * NTH == TP * NP, "a tile's elements share out evenly");
   __shared__ float2 buf[TP * PS];
   __shared__ float2 tw[NP];
   const int tid = threadIdx.x, ntiles = n2 / TP;
@@ -456,22 +474,22 @@ __global__ __launch_bounds__(pencil_threads<NP>()) void kP_fwd(const float2 *__r
   const int pq = tid % TP, j = tid / TP;
   const float2 *src = T + (size_t)k1 * n2 + c0;
   const int rs = h1 * n2;
-  float2 v[R1];
+  float2 v[F::NV];
 #pragma unroll
   for (int r = 0; r < R1; ++r) v[r] = src[(unsigned)((j + R2 * r) * rs + pq)];
-  RegFFT<NP, -1>::phase1(v, j, tw);
+  F::phase1(v, j, tw);
   float2 *pb = buf + pq * PS;
-#pragma unroll
-  for (int s = 0; s < R1; ++s) pb[xpos<R1>(s * R2 + j)] = v[s];
+  F::store_cols(v, pb, j);
   __syncthreads();
-#pragma unroll
-  for (int e = 0; e < R1; ++e) v[e] = pb[xpos<R1>(j * R1 + e)];
-  RegFFT<NP, -1>::phase2(v);
+  F::load_rows(v, pb, j);
+  F::phase2(v);
   __syncthreads();
 #pragma unroll
   for (int uu = 0; uu < ROWS; ++uu)
+    if (F::row_ok(j, uu)) {
 #pragma unroll
-    for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
+      for (int qq = 0; qq < R2; ++qq) pb[(j * ROWS + uu) + R1 * qq] = v[uu * R2 + qq];
+    }
   __syncthreads();
   float2 *dst = X + ((size_t)k1 * n2 + c0) * NP;
 #pragma unroll
@@ -509,6 +527,12 @@ template <int N1, int NC> constexpr int ana_slots() {
   constexpr int m = a > b ? a : b;
   return m > c ? m : c;
 }
+template <int N1, int NC> constexpr int ana_lds() {
+  return 8 * ana_slots<N1, NC>() + 8 * (N1 + 2 * NC) + 4 * (N1 / 2 + 1);
+}
+template <int N1, int NC> constexpr int syn_lds() {
+  return 8 * plane_slots<N1, NC>() + 8 * (N1 + 2 * NC) + 4 * (N1 / 2 + 1);
+}
 template <int N1, int NC> constexpr int ana_threads() {
   constexpr int a = (N1 / 2) * FourStep<NC>::R2, b = (NC / 2) * FourStep<N1>::R2;
   constexpr int m = a > b ? a : b;
@@ -516,7 +540,7 @@ template <int N1, int NC> constexpr int ana_threads() {
 }
 
 template <int N1, int NC, int OR>
-__global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
+__global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N1, NC>(), ana_lds<N1, NC>()>())) void kQ_ana(
     const float2 *__restrict__ slabs, int64_t H, int n1n2, NSMaskView mv, const uint32_t *__restrict__ live,
     const PlaneJob *__restrict__ jobs, float *__restrict__ out, int64_t substride, int os, float scale,
     const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg) {
@@ -524,7 +548,8 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
   constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = plane_stride<NC>(), PP = plane_stride<N1>();
   constexpr int CS = col_stride<N1>();
   constexpr int NTH = ana_threads<N1, NC>();
-  constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
+  using FC = RegFFT<NC, 1>;
+  constexpr int RC1 = FC::R1, RC2 = FC::R2, RCR = FC::ROWS;
   using F = RegFFT<N1, 1>;
   constexpr int R1 = F::R1, R2 = F::R2, ROWS = F::ROWS;
   static_assert(R1 % 2 == 0 && R1 * R2 == 2 * HS, "row HS sits at element R1 / 2 of thread 0");
@@ -583,7 +608,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     const bool act = (lrow | lhs) != 0u;
     float2 *pb = L + row * PS;
     {
-      float2 v[RC1];
+      float2 v[FC::NV];
       if (act) {
 #pragma unroll
         for (int r = 0; r < RC1; ++r)
@@ -596,15 +621,13 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
             v[r] = make_float2(v[r].x - b.y, v[r].y + b.x);
           }
         }
-        RegFFT<NC, 1>::phase1(v, j, twc);
-#pragma unroll
-        for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + j)] = v[s];
+        FC::phase1(v, j, twc);
+        FC::store_cols(v, pb, j);
       }
       wave_sync();  // the exchange region of a row is private to its R2 lanes
       if (act) {
-#pragma unroll
-        for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(j * RC1 + e)];
-        RegFFT<NC, 1>::phase2(v);
+        FC::load_rows(v, pb, j);
+        FC::phase2(v);
       }
       __syncthreads();
       if (onslot) {
@@ -613,6 +636,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
         for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
           for (int qq = 0; qq < RC2; ++qq) {
+            if (!FC::row_ok(j, uu)) continue;
             const float2 z = act ? v[uu * RC2 + qq] : make_float2(0.0f, 0.0f);
             const int kc = (j * RCR + uu) + RC1 * qq;
             float2 *lc = L + kc * CS;
@@ -630,7 +654,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     // Hermitian extension; rows that hold no data are zero.
     const int pr = tid / R2, jj = tid % R2;
     const bool on = pr < NC / 2;
-    float2 w[R1];
+    float2 w[F::NV];
     if (on) {
       const float2 *ca = L + (2 * pr) * CS, *cb = ca + CS;
       const int gz = ana_row_swz(2 * pr);
@@ -659,19 +683,18 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
     }
     __syncthreads();
     float2 *qb = L + (on ? pr : 0) * PP;
-    if (on) {
-#pragma unroll
-      for (int s = 0; s < R1; ++s) qb[xpos<R1>(s * R2 + jj)] = w[s];
-    }
+    if (on) F::store_cols(w, qb, jj);
     wave_sync();  // the exchange region of a column pair is private to its R2 lanes
     if (on) {
-#pragma unroll
-      for (int e = 0; e < R1; ++e) w[e] = qb[xpos<R1>(jj * R1 + e)];
+      F::load_rows(w, qb, jj);
       F::phase2(w);
       // Thread (pr, jj) holds columns 2 pr (.x) and 2 pr + 1 (.y) at
       // i1 = (jj ROWS + u) + R1 q.
       float *dst = out + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * os;
-      if (ROWS == 2 && os == 1 && (reinterpret_cast<uintptr_t>(dst) & 7) == 0 && (cstr & 1) == 0) {
+      if (!F::row_ok(jj, 0)) {
+        // no rows on this thread (R2 does not divide R1)
+      } else if (ROWS == 2 && R1 % 2 == 0 && os == 1 && (reinterpret_cast<uintptr_t>(dst) & 7) == 0 &&
+                 (cstr & 1) == 0) {
         float *d0 = dst + (unsigned)((2 * pr) * cstr), *d1 = d0 + cstr;
 #pragma unroll
         for (int q = 0; q < R2; ++q) {
@@ -685,6 +708,7 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), 4) void kQ_ana(
         for (int uu = 0; uu < ROWS; ++uu)
 #pragma unroll
           for (int q = 0; q < R2; ++q) {
+            if (!F::row_ok(jj, uu)) continue;
             const int i1 = (jj * ROWS + uu) + R1 * q;
             const float2 z = w[uu * R2 + q];
             dst[(int64_t)((2 * pr) * cstr + i1) * os] = z.x * scale;
@@ -723,17 +747,19 @@ __device__ __forceinline__ void plane_pairs_load(const float *__restrict__ sp, i
 // row 0 + i row n1/2 and splits the result the same way; the row n1/2 sum
 // lives in LDS (R64).
 template <int N1, int NC, int OR>
-__global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
+__global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threads<N1, NC>(), syn_lds<N1, NC>()>())) void kQ_syn(
     const float *__restrict__ in, int64_t substride, int is, int n1n2, NSMaskView mv,
     const uint32_t *__restrict__ live, const PlaneJob *__restrict__ jobs, float2 *__restrict__ slabs, int64_t H,
     const float2 *__restrict__ tw1g, const float2 *__restrict__ twcg) {
   constexpr int tpw = tp_of<NC>();  // columns per live-tile bit
   constexpr int H1 = N1 / 2 + 1, HS = N1 / 2, PS = plane_stride<NC>(), PP = plane_stride<N1>();
   constexpr int NTH = plane_threads<N1, NC>();
-  constexpr int RC1 = FourStep<NC>::R1, RC2 = FourStep<NC>::R2, RCR = RC1 / RC2;
-  constexpr int C1 = FourStep<N1>::R1, C2 = FourStep<N1>::R2, CR = C1 / C2;
+  using FR = RegFFT<NC, -1>;  // along the tile axis
+  using F1 = RegFFT<N1, -1>;  // along k1
+  constexpr int RC1 = FR::R1, RC2 = FR::R2, RCR = FR::ROWS;
+  constexpr int C1 = F1::R1, C2 = F1::R2, CR = F1::ROWS;
   static_assert(C2 % 2 == 0 && C1 * C2 == 2 * HS, "row HS sits at element (0, C2 / 2) of thread 0");
-  constexpr int NLD = N1 * NC / 2 / NTH;  // column-pair elements per thread
+  constexpr int NLD = (N1 * NC / 2 + NTH - 1) / NTH;  // column-pair elements per thread
   __shared__ __align__(16) float2 L[plane_slots<N1, NC>()];
   __shared__ float2 tw1[N1], twc[NC], R64[NC];
   __shared__ uint32_t rl[H1];
@@ -751,13 +777,14 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
   float2 *Sb = slabs + (size_t)jb.slab * H + (size_t)ip * H1 * NC;
   // Running sums in the row FFT's output order: element (uu, qq) of thread
   // (slot, j) is column kc = (j RCR + uu) + RC1 qq (row HS in R64).
-  float2 R[RC1];
+  float2 R[FR::NV];
 #pragma unroll
   for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
     for (int qq = 0; qq < RC2; ++qq) {
       const int kc = (j * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
       R[m] = make_float2(0.0f, 0.0f);
+      if (!FR::row_ok(j, uu)) continue;
       if (act && !jb.init && ((rl[slot] >> (kc / tpw)) & 1u)) R[m] = Sb[(unsigned)(slot * NC + kc)];
       if (act && slot == 0)
         R64[kc] = (!jb.init && ((rl[HS] >> (kc / tpw)) & 1u)) ? Sb[(unsigned)(HS * NC + kc)] : make_float2(0.0f, 0.0f);
@@ -769,12 +796,31 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
     const int t = opaque(tid);
     {
       // Columns (2 pr, 2 pr + 1) of the real plane -> pair pr, pair-contiguous.
-      float a[NLD], b[NLD];
-      plane_pairs_load<N1, NC, NTH, NLD>(
-          in + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * is, t, cstr, is, a, b);
-      float2 *lp = L + (t / N1) * PP + t % N1;
+      const float *sp = in + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * is;
+      if constexpr (NTH % N1 == 0 && (N1 * NC / 2) % NTH == 0) {
+        float a[NLD], b[NLD];
+        plane_pairs_load<N1, NC, NTH, NLD>(sp, t, cstr, is, a, b);
+        float2 *lp = L + (t / N1) * PP + t % N1;
 #pragma unroll
-      for (int m = 0; m < NLD; ++m) lp[m * (NTH / N1) * PP] = make_float2(a[m], b[m]);
+        for (int m = 0; m < NLD; ++m) lp[m * (NTH / N1) * PP] = make_float2(a[m], b[m]);
+      } else {
+        // Block size not a multiple of n1: element e = t + NTH m is row
+        // e % n1 of pair e / n1, each with its own offset.
+        float a[NLD], b[NLD];
+#pragma unroll
+        for (int m = 0; m < NLD; ++m) {
+          const int e = t + NTH * m, pr = e / N1, i1 = e % N1;
+          const bool in_plane = m < NLD - 1 || e < N1 * NC / 2;
+          const int o = in_plane ? (2 * pr * cstr + i1) * is : 0;
+          a[m] = sp[o];
+          b[m] = sp[o + cstr * is];
+        }
+#pragma unroll
+        for (int m = 0; m < NLD; ++m) {
+          const int e = t + NTH * m;
+          if (m < NLD - 1 || e < N1 * NC / 2) L[(e / N1) * PP + e % N1] = make_float2(a[m], b[m]);
+        }
+      }
     }
     __syncthreads();
     {
@@ -785,23 +831,19 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
       const int pr = t / C2, jj = t % C2;
       const bool on = pr < NC / 2;
       float2 *qb = L + (on ? pr : 0) * PP;
-      float2 v[C1];
+      float2 v[F1::NV];
       // The exchange region of a column pair is private to its C2 lanes.
       if (on) {
 #pragma unroll
         for (int r = 0; r < C1; ++r) v[r] = qb[jj + C2 * r];
-        RegFFT<N1, -1>::phase1(v, jj, tw1);
+        F1::phase1(v, jj, tw1);
       }
       wave_sync();
-      if (on) {
-#pragma unroll
-        for (int s = 0; s < C1; ++s) qb[xpos<C1>(s * C2 + jj)] = v[s];
-      }
+      if (on) F1::store_cols(v, qb, jj);
       wave_sync();
       if (on) {
-#pragma unroll
-        for (int e = 0; e < C1; ++e) v[e] = qb[xpos<C1>(jj * C1 + e)];
-        RegFFT<N1, -1>::phase2(v);
+        F1::load_rows(v, qb, jj);
+        F1::phase2(v);
       }
       __syncthreads();
       {
@@ -813,7 +855,7 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
 #pragma unroll
           for (int q = 0; q < C2 / 2; ++q) {
             const float2 Zm = fft_mirror<N1>(v, jj, u, q);
-            if (on) {
+            if (on && F1::row_ok(jj, u)) {
               const float2 Z = v[u * C2 + q], Zc = conjf2(Zm);
               const float2 d = csub(Z, Zc);
               const int k1 = jj * CR + u + C1 * q;
@@ -833,23 +875,19 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
       const int sl = t / RC2, jr = t % RC2;
       float2 *pb = L + (onslot ? sl : 0) * PS;
       const int cz = plane_col_swz(sl);
-      float2 v[RC1];
+      float2 v[FR::NV];
       // The exchange region of a row is private to its RC2 lanes.
       if (act) {
 #pragma unroll
         for (int r = 0; r < RC1; ++r) v[r] = pb[(jr + RC2 * r) ^ cz];
-        RegFFT<NC, -1>::phase1(v, jr, twc);
+        FR::phase1(v, jr, twc);
       }
       wave_sync();
-      if (act) {
-#pragma unroll
-        for (int s = 0; s < RC1; ++s) pb[xpos<RC1>(s * RC2 + jr)] = v[s];
-      }
+      if (act) FR::store_cols(v, pb, jr);
       wave_sync();
       if (act) {
-#pragma unroll
-        for (int e = 0; e < RC1; ++e) v[e] = pb[xpos<RC1>(jr * RC1 + e)];
-        RegFFT<NC, -1>::phase2(v);
+        FR::load_rows(v, pb, jr);
+        FR::phase2(v);
       }
       // Plane a of subband p, from row sl and column jr RCR (unit: no table).
       const float *wrow = mv.planes + (wunit ? 0 : jb.pa0 + (p - jb.p0) * jb.pa_step + sl * NC + jr * RCR);
@@ -860,7 +898,7 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
 #pragma unroll
           for (int qq = 0; qq < RC2; ++qq) {
             const float2 Zm = fft_mirror<NC>(v, jr, uu, qq);
-            if (act) {
+            if (act && FR::row_ok(jr, uu)) {
               const int kc = (jr * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
               const float2 Z = v[m], Zc = conjf2(Zm);
               const float2 F0 = cscale(cadd(Z, Zc), 0.5f);
@@ -874,12 +912,14 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
           }
       } else if (act && wunit) {
 #pragma unroll
-        for (int m = 0; m < RC1; ++m) R[m] = cadd(v[m], R[m]);
+        for (int m = 0; m < RCR * RC2; ++m)
+          if (FR::row_ok(jr, m / RC2)) R[m] = cadd(v[m], R[m]);
       } else if (act) {
 #pragma unroll
         for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
           for (int qq = 0; qq < RC2; ++qq) {
+            if (!FR::row_ok(jr, uu)) continue;
             const int m = uu * RC2 + qq;
             R[m] = fma2(v[m], wrow[uu + RC1 * qq], R[m]);
           }
@@ -892,6 +932,7 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), 4) void kQ_syn(
     for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
       for (int qq = 0; qq < RC2; ++qq) {
+        if (!FR::row_ok(j, uu)) continue;
         const int kc = (j * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
         if ((rl[slot] >> (kc / tpw)) & 1u) Sb[(unsigned)(slot * NC + kc)] = R[m];
         if (slot == 0 && ((rl[HS] >> (kc / tpw)) & 1u)) Sb[(unsigned)(HS * NC + kc)] = R64[kc];
@@ -948,20 +989,36 @@ template <int N1, int NC, int OR> struct PlaneL {
   }
 };
 
+template <int N1, int NC, int OR> bool set_plane_n(FusedOps &o) {
+  o.q_ana[OR] = &PlaneL<N1, NC, OR>::ana;
+  o.q_syn[OR] = &PlaneL<N1, NC, OR>::syn;
+  return true;
+}
 template <int N1, int OR> bool set_plane(int nc, FusedOps &o) {
   switch (nc) {
-    case 32: o.q_ana[OR] = &PlaneL<N1, 32, OR>::ana; o.q_syn[OR] = &PlaneL<N1, 32, OR>::syn; return true;
-    case 64: o.q_ana[OR] = &PlaneL<N1, 64, OR>::ana; o.q_syn[OR] = &PlaneL<N1, 64, OR>::syn; return true;
-    case 128: o.q_ana[OR] = &PlaneL<N1, 128, OR>::ana; o.q_syn[OR] = &PlaneL<N1, 128, OR>::syn; return true;
+    case 32: return set_plane_n<N1, 32, OR>(o);
+    case 64: return set_plane_n<N1, 64, OR>(o);
+    case 80: return set_plane_n<N1, 80, OR>(o);
+    case 96: return set_plane_n<N1, 96, OR>(o);
+    case 112: return set_plane_n<N1, 112, OR>(o);
+    case 128: return set_plane_n<N1, 128, OR>(o);
     default: return false;
   }
 }
 
+template <int NP, int OR> bool set_pencil_n(FusedOps &o) {
+  o.p_ana[OR] = &PencilL<NP, OR>::ana;
+  o.p_syn[OR] = &PencilL<NP, OR>::syn;
+  return true;
+}
 template <int OR> bool set_pencil(int np, FusedOps &o) {
   switch (np) {
-    case 32: o.p_ana[OR] = &PencilL<32, OR>::ana; o.p_syn[OR] = &PencilL<32, OR>::syn; return true;
-    case 64: o.p_ana[OR] = &PencilL<64, OR>::ana; o.p_syn[OR] = &PencilL<64, OR>::syn; return true;
-    case 128: o.p_ana[OR] = &PencilL<128, OR>::ana; o.p_syn[OR] = &PencilL<128, OR>::syn; return true;
+    case 32: return set_pencil_n<32, OR>(o);
+    case 64: return set_pencil_n<64, OR>(o);
+    case 80: return set_pencil_n<80, OR>(o);
+    case 96: return set_pencil_n<96, OR>(o);
+    case 112: return set_pencil_n<112, OR>(o);
+    case 128: return set_pencil_n<128, OR>(o);
     default: return false;
   }
 }
@@ -974,17 +1031,23 @@ template <int N1> bool set_planes(int n2, int n3, FusedOps &o) {
 
 bool fused_ops_for(int n1, int n2, int n3, FusedOps &o) {
   o = FusedOps{};
-  auto pow2 = [](int n) { return n == 32 || n == 64 || n == 128; };
-  if (!pow2(n1) || !pow2(n2) || !pow2(n3)) return false;
+  auto fits = [](int n) { return n == 32 || n == 64 || n == 80 || n == 96 || n == 112 || n == 128; };
+  if (!fits(n1) || !fits(n2) || !fits(n3)) return false;
   bool ok = set_pencil<AX3>(n3, o) && set_pencil<AX2>(n2, o);
   switch (n1) {
     case 32: ok = ok && set_planes<32>(n2, n3, o); break;
     case 64: ok = ok && set_planes<64>(n2, n3, o); break;
+    case 80: ok = ok && set_planes<80>(n2, n3, o); break;
+    case 96: ok = ok && set_planes<96>(n2, n3, o); break;
+    case 112: ok = ok && set_planes<112>(n2, n3, o); break;
     default: ok = ok && set_planes<128>(n2, n3, o); break;
   }
   switch (n3) {
     case 32: o.p_fwd = &launch_fwd<32>; break;
     case 64: o.p_fwd = &launch_fwd<64>; break;
+    case 80: o.p_fwd = &launch_fwd<80>; break;
+    case 96: o.p_fwd = &launch_fwd<96>; break;
+    case 112: o.p_fwd = &launch_fwd<112>; break;
     default: o.p_fwd = &launch_fwd<128>; break;
   }
   o.tile = tp_of<32>();

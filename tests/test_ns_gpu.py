@@ -64,6 +64,12 @@ CASES = [
     # workload's transform: factored and general kernels
     ((64, 64, 64), 4, (2, 2, 3, 3), 12, True),
     ((72, 80, 64), 4, (2, 2, 3, 3), 12, True),
+    # the factored kernels at lengths 80, 96, 112 (8 threads x 10, 12, 14
+    # points per pencil, uneven phase-2 rows), mixed with the power-of-two
+    # lengths and with each other
+    ((96, 80, 112), 3, (2, 2, 3), 12, True),
+    ((112, 96, 32), 2, (1, 2), 6, False),
+    ((80, 128, 96), 2, (2, 3), 12, True),
 ]
 
 
@@ -242,13 +248,17 @@ def test_ns_shift_equivariance(n):
     assert np.max(np.abs(b - np.roll(a, s, axis=(0, 1, 2)))) < 1e-5
 
 
-@pytest.mark.parametrize("dims", [(96, 96, 96), (128, 128, 128)])
-def test_ns_default_transform_matches_oracle(dims):
+@pytest.mark.parametrize("dims,route", [((96, 96, 96), "factored"), ((96, 96, 96), "general"),
+                                        ((128, 128, 128), "factored")])
+def test_ns_default_transform_matches_oracle(dims, route, monkeypatch):
     # NSSurfacelet(scales = 3, direction_levels = [2, 2, 3]), 49 subbands, on
-    # Float32 volumes at full size: 96^3 (the reference's CPU-runnable case,
-    # general-size kernels) and 128^3 (the headline, factored kernels).
+    # Float32 volumes at full size: 96^3 (the reference's CPU-runnable case)
+    # on the factored kernels and, forced, on the general-size kernels, and
+    # 128^3 (the headline).
     dev = _dev()
-    print(f"case {dims} default transform", flush=True)
+    print(f"case {dims} default transform, {route} kernels", flush=True)
+    if route == "general":
+        monkeypatch.setenv("SURFHIP_NS_GENERIC", "1")
     x = np.random.default_rng(20260).standard_normal(dims).astype(np.float32)
     t = N.NSSurfacelet(scales=3, direction_levels=[2, 2, 3])
     to = O.NSParams(3, (2, 2, 3))
@@ -269,13 +279,35 @@ def test_ns_default_transform_matches_oracle(dims):
     # synthesis of the same coefficients against the oracle's synthesis
     yref = O.synthesize(to, [cn[..., i] for i in range(cn.shape[-1])], tb)
     assert _rel(to_numpy(xr), yref) <= RTOL
-    print(f"{dims}: analysis rel L2 vs oracle {ea:.2e}, synthesis vs oracle {_rel(to_numpy(xr), yref):.2e}, "
+    print(f"{dims} {route}: analysis rel L2 vs oracle {ea:.2e}, synthesis vs oracle {_rel(to_numpy(xr), yref):.2e}, "
           f"reconstruction {_rel(to_numpy(xr), x):.2e}", flush=True)
     del cn, yref
     # streaming at full size equals the materialised transform bit for bit
     y2 = torch.empty_like(xr)
     N.synthesize_(lambda i: N.subband(t, c, i), y2, t, workspace=ws)
     assert torch.equal(xr, y2)
+
+
+@pytest.mark.parametrize("dims,factored", [((96, 96, 96), True), ((112, 128, 96), True), ((80, 32, 112), True),
+                                          ((128, 64, 32), True), ((80, 96, 72), False), ((48, 48, 48), False)])
+def test_ns_kernel_route_by_size(dims, factored, monkeypatch):
+    """Grids whose three lengths are in {32, 64, 80, 96, 112, 128} run the
+    factored kernels (plane and pencil classes); others, or any grid with
+    SURFHIP_NS_GENERIC set, the general-size kernels (rows and axes classes)."""
+    dev = _dev()
+    t = N.NSSurfacelet(scales=2, direction_levels=[1, 2], bernstein_order=6)
+    x = from_numpy(np.random.default_rng(5).standard_normal(dims).astype(np.float32), dev)
+    for generic in (False, True):
+        if generic:
+            monkeypatch.setenv("SURFHIP_NS_GENERIC", "1")
+        ws = N.make_workspace(t, x, prewarm=False)
+        ws.set_timing(True)
+        N.synthesize(ws, N.analyze(ws, x))
+        cls = set(ws.timing())
+        ws.set_timing(False)
+        assert ("plane_inverse" in cls) == (factored and not generic), (dims, generic, cls)
+        assert ("rows_c2r" in cls) == (not factored or generic), (dims, generic, cls)
+    monkeypatch.delenv("SURFHIP_NS_GENERIC")
 
 
 @pytest.mark.parametrize("dims,J,L,tight", [((64, 32, 128), 3, (1, 2, 3), True), ((32, 64, 64), 2, (2, 3), False)])
