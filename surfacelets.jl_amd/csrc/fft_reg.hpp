@@ -96,12 +96,14 @@ template <int N, int DIR> struct RegFFT {
   static constexpr int NV = ROWS * R2 > R1 ? ROWS * R2 : R1;
   // Whether row u of thread j exists (always when R2 divides R1).
   __device__ static __forceinline__ bool row_ok(int j, int u) { return R1 % R2 == 0 || j * ROWS + u < R1; }
-  // Exchange slot of element e: one pad slot per R1 slots, so that the R1
-  // consecutive slots each thread reads in phase 2 start in different banks;
-  // when R2 does not divide R1, one pad slot per row of R2 (a shift, where
-  // e / R1 would be a division).  A pencil's exchange spans N + PAD slots.
-  static constexpr int PAD = R1 % R2 == 0 ? N / R1 : N / R2;
-  __device__ static __forceinline__ int xpos(int e) { return R1 % R2 == 0 ? e + e / R1 : e + e / R2; }
+  // Exchange slot of element e: one pad slot per ROWS R2 slots (the rows one
+  // thread reads in phase 2; R1 slots when R2 divides R1), so that the
+  // threads' rows start in different banks.  ROWS R2 is a power of two for
+  // every length here, so e / (ROWS R2) is a shift.  A pencil's exchange spans
+  // N + PAD slots.
+  static constexpr int PADDIV = ROWS * R2;
+  static constexpr int PAD = (N + PADDIV - 1) / PADDIV;
+  __device__ static __forceinline__ int xpos(int e) { return e + e / PADDIV; }
   // Phase 1 on values already in registers (v[r] = x[j + R2 r]).  twT is the
   // twiddle table in phase-1 order (fill_twiddles): thread j reads
   // twT[s R2 + j] = w_N^(j s), one base address and constant offsets.
@@ -125,18 +127,14 @@ template <int N, int DIR> struct RegFFT {
   // the rows thread j holds (the R1 consecutive slots from j R1 when R2
   // divides R1).
   __device__ static __forceinline__ void load_rows(float2 (&v)[NV], const float2 *pb, int j) {
-    if constexpr (R1 % R2 == 0) {
+    // Thread j's rows are the ROWS R2 consecutive slots from j ROWS R2, one
+    // pad unit: they sit at j (ROWS R2 + 1) + u R2 + q.
 #pragma unroll
-      for (int e = 0; e < R1; ++e) v[e] = pb[xpos(j * R1 + e)];
-    } else {
+    for (int u = 0; u < ROWS; ++u)
+      if (row_ok(j, u)) {
 #pragma unroll
-      for (int u = 0; u < ROWS; ++u)
-        if (row_ok(j, u)) {
-          const float2 *row = pb + (j * ROWS + u) * (R2 + 1);  // slot s R2 + q sits at s (R2 + 1) + q
-#pragma unroll
-          for (int q = 0; q < R2; ++q) v[u * R2 + q] = row[q];
-        }
-    }
+        for (int q = 0; q < R2; ++q) v[u * R2 + q] = pb[j * (PADDIV + 1) + u * R2 + q];
+      }
   }
   // Phase 2 on v[u R2 + q'] = slot((j ROWS + u) R2 + q'); results in place:
   // v[u R2 + q] = X[(j ROWS + u) + R1 q].

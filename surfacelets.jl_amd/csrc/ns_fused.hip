@@ -36,22 +36,44 @@ namespace surfhip {
 // Pencils per tile and threads per pencil block (R2 threads per pencil).
 template <int N> constexpr int tp_of() { return 16; }
 template <int N> constexpr int pencil_threads() { return tp_of<N>() * FourStep<N>::R2; }
-// LDS slots of one pencil: the padded exchange needs N + N/R1, and the
+// LDS slots of one pencil: the padded exchange needs N + PAD, and the
 // stride spreads the eight pencils that 16 lanes touch at once over the
-// banks: for N = 128 (8 lanes per pencil) 138 slots = 10 mod 16 makes the
+// banks: for the lengths with 8 lanes per pencil and two rows per thread
+// (80, 96, 112, 128) a stride of 6 or 10 mod 16 (138 at 128) makes the
 // loads and both sides of the exchange conflict-free; other lengths take
 // the next odd stride.
-template <int N> constexpr int pencil_stride() { return N == 128 ? 138 : N + RegFFT<N, 1>::PAD + 1; }
+template <int N> constexpr int pencil_stride() {
+  constexpr int need = N + RegFFT<N, 1>::PAD;
+  if constexpr (N == 128 || FourStep<N>::R1 % FourStep<N>::R2 != 0) {
+    // 10 or 6 mod 16 (the lengths with 8 lanes per pencil and two rows per
+    // thread): loads and both sides of the exchange conflict-free
+    int p = need;
+    while (p % 16 != 6 && p % 16 != 10) ++p;
+    return p;
+  } else {
+    return need + 1;
+  }
+}
 
 // Plane blocks: n1/2 row slots of the tile axis (slot 0 carries rows 0 and
 // n1/2), the pair-contiguous columns of the axis-1 transform, the real output.
-// A row or column pair takes N + N/R1 slots (the padded exchange): for
-// N = 128 that is 8 mod 16, so the two rows that 16 lanes of an 8-byte LDS
-// access touch fall in opposite halves of the 32 banks.  The transposed
+// A row or column pair takes N + PAD slots (the padded exchange), rounded up
+// to 8 mod 16 for 8 lanes per line (136 at 128), so the two rows that 16
+// lanes of an 8-byte LDS access touch fall in opposite halves of the 32
+// banks.  The transposed
 // layouts written between the two passes are XOR-swizzled (plane_col_swz,
 // ana_row_swz) instead of padded, so every wide access of the 128 x 128
 // plane kernels is free of bank conflicts.
-template <int N> constexpr int plane_stride() { return N + RegFFT<N, 1>::PAD; }
+template <int N> constexpr int plane_stride() {
+  constexpr int need = N + RegFFT<N, 1>::PAD;
+  if constexpr (FourStep<N>::R2 == 8) {
+    int p = need;  // 8 mod 16 (see above)
+    while (p % 16 != 8) ++p;
+    return p;
+  } else {
+    return need;
+  }
+}
 template <int N1, int NC> constexpr int plane_slots() {
   constexpr int a = (N1 / 2) * plane_stride<NC>(), b = (NC / 2) * plane_stride<N1>(), c = N1 * NC / 2;
   constexpr int m = a > b ? a : b;
