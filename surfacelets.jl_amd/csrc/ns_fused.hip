@@ -593,16 +593,23 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
   // HS: the inverse along kc of each row is real (their spectra are
   // Hermitian in kc), so the real part is row 0's result and the imaginary
   // part row HS's.
+  // Only live tiles are read: the pencil pass writes nothing else, and the
+  // row pass drops dead tiles whatever they hold.
   float2 u[RC1];
   float2 uh = make_float2(0.0f, 0.0f);
   {
     const int tid = (int)threadIdx.x;
     const int slot = tid / RC2, j = tid % RC2;
     const int row = slot < HS ? slot : 0;
+    const uint32_t lw = slot >= HS ? 0u : masked ? live[(size_t)jb.slab * H1 + row] : 0xffffffffu;
 #pragma unroll
-    for (int r = 0; r < RC1; ++r) u[r] = Ub[(unsigned)(row * NC + j + RC2 * r)];
+    for (int r = 0; r < RC1; ++r) {
+      u[r] = make_float2(0.0f, 0.0f);
+      if ((lw >> ((j + RC2 * r) / tpw)) & 1u) u[r] = Ub[(unsigned)(row * NC + j + RC2 * r)];
+    }
     static_assert(NTH >= NC, "one thread per column of row HS");
-    if (tid < NC) uh = Ub[(unsigned)(HS * NC + tid)];
+    const uint32_t lh = masked ? live[(size_t)jb.slab * H1 + HS] : 0xffffffffu;
+    if (tid < NC && ((lh >> (tid / tpw)) & 1u)) uh = Ub[(unsigned)(HS * NC + tid)];
   }
   for (int p = jb.p0; p < jb.p1; ++p) {
     // Index math is redone per subband from an opaque copy of the thread
