@@ -124,13 +124,6 @@ template <class T> __device__ __forceinline__ T &at_b(T *base, unsigned bytes) {
   return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + bytes);
 }
 
-// a where bit is 1, else +0: the bit patterns are masked, so there is no
-// branch and whatever a holds (NaN included) is dropped.
-__device__ __forceinline__ float2 keep(float2 a, uint32_t bit) {
-  const uint32_t m = 0u - bit;
-  return make_float2(__uint_as_float(__float_as_uint(a.x) & m), __uint_as_float(__float_as_uint(a.y) & m));
-}
-
 __device__ __forceinline__ float2 fma2(float2 a, float s, float2 c) {
   return make_float2(__fmaf_rn(a.x, s, c.x), __fmaf_rn(a.y, s, c.y));
 }
@@ -639,14 +632,14 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
     {
       float2 v[FC::NV];
       if (act) {
+        // Dead tiles hold +0: their slab values were never loaded (u = 0, uh
+        // = 0) and the wedge values are finite and nonnegative.
 #pragma unroll
-        for (int r = 0; r < RC1; ++r)
-          v[r] = keep(cscale(u[r], wunit ? 1.0f : wa[r]), (lrow >> ((j + RC2 * r) / tpw)) & 1u);
+        for (int r = 0; r < RC1; ++r) v[r] = cscale(u[r], wunit ? 1.0f : wa[r]);
         if (slot == 0) {
 #pragma unroll
           for (int r = 0; r < RC1; ++r) {
-            const int kc = j + RC2 * r;
-            const float2 b = keep(UW[kc], (lhs >> (kc / tpw)) & 1u);
+            const float2 b = UW[j + RC2 * r];
             v[r] = make_float2(v[r].x - b.y, v[r].y + b.x);
           }
         }
@@ -724,13 +717,14 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
         // no rows on this thread (R2 does not divide R1)
       } else if (ROWS == 2 && R1 % 2 == 0 && os == 1 && (reinterpret_cast<uintptr_t>(dst) & 7) == 0 &&
                  (cstr & 1) == 0) {
-        float *d0 = dst + (unsigned)((2 * pr) * cstr), *d1 = d0 + cstr;
+        // Block-uniform base, 32-bit byte offsets per thread.
+        float2 *db = reinterpret_cast<float2 *>(dst);
+        const unsigned o0 = 4u * (unsigned)((2 * pr) * cstr + jj * 2), o1 = o0 + 4u * (unsigned)cstr;
 #pragma unroll
         for (int q = 0; q < R2; ++q) {
-          const int i1 = jj * 2 + R1 * q;
           const float2 z0 = w[q], z1 = w[R2 + q];
-          *reinterpret_cast<float2 *>(d0 + i1) = make_float2(z0.x * scale, z1.x * scale);
-          *reinterpret_cast<float2 *>(d1 + i1) = make_float2(z0.y * scale, z1.y * scale);
+          at_b(db, o0 + 4u * R1 * q) = make_float2(z0.x * scale, z1.x * scale);
+          at_b(db, o1 + 4u * R1 * q) = make_float2(z0.y * scale, z1.y * scale);
         }
       } else {
 #pragma unroll
