@@ -115,6 +115,17 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
+// The thread index rebuilt where it is used, from the wave's first index
+// (w64, wave-uniform: an SGPR) and the lane id (v_mbcnt in a volatile asm,
+// so it is recomputed rather than kept live): no VGPR holds the index across
+// a loop.
+__device__ __forceinline__ int thread_index(int w64) {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return w64 + l;
+}
+__device__ __forceinline__ int wave_base() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x & ~63); }
+
 // base[byte offset]: a block-uniform base plus a 32-bit byte offset, the
 // form the scalar-base global loads and stores take.
 template <class T> __device__ __forceinline__ const T &at_b(const T *base, unsigned bytes) {
@@ -815,8 +826,9 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
   const int pbase = OR == AX3 ? ip * n1n2 : ip * N1;
   const int cstr = OR == AX3 ? N1 : n1n2;
   const bool wunit = !masked || jb.pa0 < 0 || ns_unit_wedge(mv, false);
+  const int w64 = wave_base();
   for (int p = jb.p0; p < jb.p1; ++p) {
-    const int t = opaque(tid);
+    const int t = thread_index(w64);
     {
       // Columns (2 pr, 2 pr + 1) of the real plane -> pair pr, pair-contiguous.
       const float *sp = in + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * is;
