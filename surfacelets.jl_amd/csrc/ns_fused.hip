@@ -796,7 +796,14 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
   constexpr int NLD = (N1 * NC / 2 + NTH - 1) / NTH;  // column-pair elements per thread
   __shared__ __align__(16) float2 L[plane_slots<N1, NC>()];
   __shared__ float2 tw1[N1], twc[NC], R64[NC];
+  // Rows 0 and HS of the subband's plane a (1 when unit), for slot 0: staged
+  // in LDS where the kernel's registers allow it (W01R), so that slot 0 does
+  // not wait on one global load per element; at 128 x 128 the staging costs
+  // spills inside the subband loop, and slot 0 reads the table directly.
+  constexpr bool W01R = !(N1 == 128 && NC == 128);
+  __shared__ float W01[W01R ? 2 * NC : 1];
   __shared__ uint32_t rl[H1];
+  static_assert(!W01R || NTH >= 2 * NC, "one thread per column of rows 0 and HS");
   const int tid = threadIdx.x, ip = blockIdx.x;
   const PlaneJob jb = jobs[blockIdx.y];
   const bool masked = jb.group >= 0;
@@ -829,6 +836,10 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
   const int w64 = wave_base();
   for (int p = jb.p0; p < jb.p1; ++p) {
     const int t = thread_index(w64);
+    // (Two barriers before their use; the previous subband's reads ended at
+    // the barrier closing its iteration.)
+    if (W01R && t < 2 * NC)
+      W01[t] = wunit ? 1.0f : mv.planes[jb.pa0 + (p - jb.p0) * jb.pa_step + (t < NC ? t : HS * NC + t - NC)];
     {
       // Columns (2 pr, 2 pr + 1) of the real plane -> pair pr, pair-contiguous.
       const float *sp = in + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * is;
@@ -939,8 +950,8 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
               const float2 F0 = cscale(cadd(Z, Zc), 0.5f);
               const float2 d = csub(Z, Zc);
               const float2 F1 = make_float2(0.5f * d.y, -0.5f * d.x);
-              const float w0 = wunit ? 1.0f : wrow[uu + RC1 * qq];
-              const float w1 = wunit ? 1.0f : wrow[HS * NC + uu + RC1 * qq];
+              const float w0 = W01R ? W01[kc] : wunit ? 1.0f : wrow[uu + RC1 * qq];
+              const float w1 = W01R ? W01[NC + kc] : wunit ? 1.0f : wrow[HS * NC + uu + RC1 * qq];
               R[m] = fma2(F0, w0, R[m]);
               R64[kc] = fma2(F1, w1, R64[kc]);
             }
