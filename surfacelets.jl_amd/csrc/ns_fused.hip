@@ -557,7 +557,7 @@ template <int N1, int NC> constexpr int ana_lds() {
   return 8 * ana_slots<N1, NC>() + 8 * (N1 + 2 * NC) + 4 * (N1 / 2 + 1);
 }
 template <int N1, int NC> constexpr int syn_lds() {
-  return 8 * plane_slots<N1, NC>() + 8 * (N1 + 2 * NC) + 4 * (N1 / 2 + 1);
+  return 8 * plane_slots<N1, NC>() + 8 * (N1 + 4 * NC) + 4 * (N1 / 2 + 1);
 }
 template <int N1, int NC> constexpr int ana_threads() {
   constexpr int a = (N1 / 2) * FourStep<NC>::R2, b = (NC / 2) * FourStep<N1>::R2;
@@ -778,8 +778,12 @@ __device__ __forceinline__ void plane_pairs_load(const float *__restrict__ sp, i
 // (rows k1 <= n1/2, live tiles).  The transform along axis 1 runs on column
 // pairs and is unpacked in registers (the mirror Z[-k] comes from a lane of
 // the same pencil).  Rows 0 and n1/2 of r2c_1 are real, so slot 0 transforms
-// row 0 + i row n1/2 and splits the result the same way; the row n1/2 sum
-// lives in LDS (R64).
+// z = row 0 + i row n1/2 and only parks the result Z in LDS (ZR); the split
+// F(row 0) = (Z[kc] + conj Z[-kc]) / 2, F(row n1/2) = (Z[kc] - conj Z[-kc]) / 2i
+// and the two masked sums (R0, R64, in LDS, natural column order) are done
+// by threads kc < NC, one column each, at the start of the next subband
+// while its plane loads are in flight (and once after the last subband).
+// Slot 0's own lanes therefore do no more work than any other row's.
 template <int N1, int NC, int OR>
 __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threads<N1, NC>(), syn_lds<N1, NC>()>())) void kQ_syn(
     const float *__restrict__ in, int64_t substride, int is, int n1n2, NSMaskView mv,
@@ -793,17 +797,11 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
   constexpr int RC1 = FR::R1, RC2 = FR::R2, RCR = FR::ROWS;
   constexpr int C1 = F1::R1, C2 = F1::R2, CR = F1::ROWS;
   static_assert(C2 % 2 == 0 && C1 * C2 == 2 * HS, "row HS sits at element (0, C2 / 2) of thread 0");
+  static_assert(NTH >= NC, "one thread per column of rows 0 and HS");
   constexpr int NLD = (N1 * NC / 2 + NTH - 1) / NTH;  // column-pair elements per thread
   __shared__ __align__(16) float2 L[plane_slots<N1, NC>()];
-  __shared__ float2 tw1[N1], twc[NC], R64[NC];
-  // Rows 0 and HS of the subband's plane a (1 when unit), for slot 0: staged
-  // in LDS where the kernel's registers allow it (W01R), so that slot 0 does
-  // not wait on one global load per element; at 128 x 128 the staging costs
-  // spills inside the subband loop, and slot 0 reads the table directly.
-  constexpr bool W01R = !(N1 == 128 && NC == 128);
-  __shared__ float W01[W01R ? 2 * NC : 1];
+  __shared__ float2 tw1[N1], twc[NC], R0[NC], R64[NC], ZR[NC];
   __shared__ uint32_t rl[H1];
-  static_assert(!W01R || NTH >= 2 * NC, "one thread per column of rows 0 and HS");
   const int tid = threadIdx.x, ip = blockIdx.x;
   const PlaneJob jb = jobs[blockIdx.y];
   const bool masked = jb.group >= 0;
@@ -813,11 +811,14 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
   __syncthreads();
   const int slot = tid / RC2, j = tid % RC2;
   const bool onslot = slot < HS;
-  const uint32_t lrow = onslot ? (slot == 0 ? (rl[0] | rl[HS]) : rl[slot]) : 0u;
+  // Slot 0 runs (block-uniform) when row 0 or row HS has a live tile.
+  const bool act0 = (rl[0] | rl[HS]) != 0u;
+  const uint32_t lrow = onslot ? (slot == 0 ? (act0 ? 1u : 0u) : rl[slot]) : 0u;
   const bool act = lrow != 0u;
   float2 *Sb = slabs + (size_t)jb.slab * H + (size_t)ip * H1 * NC;
-  // Running sums in the row FFT's output order: element (uu, qq) of thread
-  // (slot, j) is column kc = (j RCR + uu) + RC1 qq (row HS in R64).
+  // Running sums of rows 1 .. HS - 1 in the row FFT's output order: element
+  // (uu, qq) of thread (slot, j) is column kc = (j RCR + uu) + RC1 qq.  Rows 0
+  // and HS sum in R0 and R64 (thread kc < NC owns column kc).
   float2 R[FR::NV];
 #pragma unroll
   for (int uu = 0; uu < RCR; ++uu)
@@ -825,21 +826,41 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
     for (int qq = 0; qq < RC2; ++qq) {
       const int kc = (j * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
       R[m] = make_float2(0.0f, 0.0f);
-      if (!FR::row_ok(j, uu)) continue;
+      if (!FR::row_ok(j, uu) || slot == 0) continue;
       if (act && !jb.init && ((rl[slot] >> (kc / tpw)) & 1u)) R[m] = Sb[(unsigned)(slot * NC + kc)];
-      if (act && slot == 0)
-        R64[kc] = (!jb.init && ((rl[HS] >> (kc / tpw)) & 1u)) ? Sb[(unsigned)(HS * NC + kc)] : make_float2(0.0f, 0.0f);
     }
+  if (act0 && tid < NC) {
+    const bool ld = !jb.init;
+    R0[tid] = ld && ((rl[0] >> (tid / tpw)) & 1u) ? Sb[(unsigned)tid] : make_float2(0.0f, 0.0f);
+    R64[tid] = ld && ((rl[HS] >> (tid / tpw)) & 1u) ? Sb[(unsigned)(HS * NC + tid)] : make_float2(0.0f, 0.0f);
+  }
   const int pbase = OR == AX3 ? ip * n1n2 : ip * N1;
   const int cstr = OR == AX3 ? N1 : n1n2;
   const bool wunit = !masked || jb.pa0 < 0 || ns_unit_wedge(mv, false);
   const int w64 = wave_base();
+  // Rows 0 and HS of subband pp from ZR (written by slot 0 before the barrier
+  // that closed pp's iteration; rewritten only two barriers into the next),
+  // with w0, w1 = plane a of pp at (0, kc) and (HS, kc).
+  auto split_w = [&](int pp, int kc, float &w0, float &w1) {
+    const float *wa = mv.planes + (wunit ? 0 : jb.pa0 + (pp - jb.p0) * jb.pa_step);
+    w0 = wunit ? 1.0f : wa[kc];
+    w1 = wunit ? 1.0f : wa[HS * NC + kc];
+  };
+  auto split0 = [&](int kc, float w0, float w1) {
+    const float2 Z = ZR[kc], Zc = conjf2(ZR[(NC - kc) % NC]);
+    const float2 F0 = cscale(cadd(Z, Zc), 0.5f);
+    const float2 d = csub(Z, Zc);
+    const float2 F1 = make_float2(0.5f * d.y, -0.5f * d.x);
+    R0[kc] = fma2(F0, w0, R0[kc]);
+    R64[kc] = fma2(F1, w1, R64[kc]);
+  };
   for (int p = jb.p0; p < jb.p1; ++p) {
     const int t = thread_index(w64);
-    // (Two barriers before their use; the previous subband's reads ended at
-    // the barrier closing its iteration.)
-    if (W01R && t < 2 * NC)
-      W01[t] = wunit ? 1.0f : mv.planes[jb.pa0 + (p - jb.p0) * jb.pa_step + (t < NC ? t : HS * NC + t - NC)];
+    // The split of the previous subband: its wedge values are loaded ahead of
+    // the plane, the split itself runs once the plane is in LDS.
+    const bool sp = act0 && p > jb.p0 && t < NC;
+    float w0 = 1.0f, w1 = 1.0f;
+    if (sp) split_w(p - 1, t, w0, w1);
     {
       // Columns (2 pr, 2 pr + 1) of the real plane -> pair pr, pair-contiguous.
       const float *sp = in + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * is;
@@ -868,6 +889,7 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
         }
       }
     }
+    if (sp) split0(t, w0, w1);
     __syncthreads();
     {
       // FFT along k1 of z = col 2 pr + i col 2 pr + 1, then rows k <= HS:
@@ -937,25 +959,15 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
       }
       // Plane a of subband p, from row sl and column jr RCR (unit: no table).
       const float *wrow = mv.planes + (wunit ? 0 : jb.pa0 + (p - jb.p0) * jb.pa_step + sl * NC + jr * RCR);
-      if (slot == 0) {
-        // Z = F(row 0) + i F(row HS), both rows real: split through Z[-kc].
+      if (sl == 0) {
+        // Z = F(row 0) + i F(row HS), parked for the split.
+        if (act) {
 #pragma unroll
-        for (int uu = 0; uu < RCR; ++uu)
+          for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
-          for (int qq = 0; qq < RC2; ++qq) {
-            const float2 Zm = fft_mirror<NC>(v, jr, uu, qq);
-            if (act && FR::row_ok(jr, uu)) {
-              const int kc = (jr * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
-              const float2 Z = v[m], Zc = conjf2(Zm);
-              const float2 F0 = cscale(cadd(Z, Zc), 0.5f);
-              const float2 d = csub(Z, Zc);
-              const float2 F1 = make_float2(0.5f * d.y, -0.5f * d.x);
-              const float w0 = W01R ? W01[kc] : wunit ? 1.0f : wrow[uu + RC1 * qq];
-              const float w1 = W01R ? W01[NC + kc] : wunit ? 1.0f : wrow[HS * NC + uu + RC1 * qq];
-              R[m] = fma2(F0, w0, R[m]);
-              R64[kc] = fma2(F1, w1, R64[kc]);
-            }
-          }
+            for (int qq = 0; qq < RC2; ++qq)
+              if (FR::row_ok(jr, uu)) ZR[(jr * RCR + uu) + RC1 * qq] = v[uu * RC2 + qq];
+        }
       } else if (act && wunit) {
 #pragma unroll
         for (int m = 0; m < RCR * RC2; ++m)
@@ -971,9 +983,19 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
           }
       }
     }
-    __syncthreads();  // L is reloaded by the next subband
+    __syncthreads();  // L is reloaded by the next subband; ZR is complete
   }
-  if (act) {
+  {
+    const int t = thread_index(w64);
+    if (act0 && t < NC) {
+      float w0, w1;
+      split_w(jb.p1 - 1, t, w0, w1);
+      split0(t, w0, w1);
+      if ((rl[0] >> (t / tpw)) & 1u) Sb[(unsigned)t] = R0[t];
+      if ((rl[HS] >> (t / tpw)) & 1u) Sb[(unsigned)(HS * NC + t)] = R64[t];
+    }
+  }
+  if (act && slot != 0) {
 #pragma unroll
     for (int uu = 0; uu < RCR; ++uu)
 #pragma unroll
@@ -981,7 +1003,6 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
         if (!FR::row_ok(j, uu)) continue;
         const int kc = (j * RCR + uu) + RC1 * qq, m = uu * RC2 + qq;
         if ((rl[slot] >> (kc / tpw)) & 1u) Sb[(unsigned)(slot * NC + kc)] = R[m];
-        if (slot == 0 && ((rl[HS] >> (kc / tpw)) & 1u)) Sb[(unsigned)(HS * NC + kc)] = R64[kc];
       }
   }
 }
