@@ -799,6 +799,10 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
   static_assert(C2 % 2 == 0 && C1 * C2 == 2 * HS, "row HS sits at element (0, C2 / 2) of thread 0");
   static_assert(NTH >= NC, "one thread per column of rows 0 and HS");
   constexpr int NLD = (N1 * NC / 2 + NTH - 1) / NTH;  // column-pair elements per thread
+  // Whether every column-pass lane is a thread (NTH = NC / 2 pairs x C2) and
+  // a pair's rows split into whole 64-row runs: then each wave stages only
+  // the pairs it transforms.
+  constexpr bool WAVEIN = NTH == (NC / 2) * C2 && N1 % 64 == 0 && 64 % C2 == 0;
   __shared__ __align__(16) float2 L[plane_slots<N1, NC>()];
   __shared__ float2 tw1[N1], twc[NC], R0[NC], R64[NC], ZR[NC];
   __shared__ uint32_t rl[H1];
@@ -864,7 +868,22 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
     {
       // Columns (2 pr, 2 pr + 1) of the real plane -> pair pr, pair-contiguous.
       const float *sp = in + (int64_t)(jb.out0 + (p - jb.p0) * jb.out_step) * substride + (int64_t)pbase * is;
-      if constexpr (NTH % N1 == 0 && (N1 * NC / 2) % NTH == 0) {
+      if constexpr (WAVEIN) {
+        // Each wave loads the column pairs its own lanes transform (64 / C2
+        // pairs, one 64-row run of one pair per load), so the staging in L
+        // needs only a wave-level fence, not a block barrier.
+        float a[C1], b[C1];
+        const int lane = t - w64, pw = w64 / C2;
+#pragma unroll
+        for (int m = 0; m < C1; ++m) {
+          const int pr = pw + (64 * m) / N1, r0 = (64 * m) % N1;
+          const float *col = sp + (int64_t)(2 * pr * cstr + r0) * is;
+          a[m] = at_b(col, 4u * (unsigned)(lane * is));
+          b[m] = at_b(col, 4u * (unsigned)((cstr + lane) * is));
+        }
+#pragma unroll
+        for (int m = 0; m < C1; ++m) L[(pw + (64 * m) / N1) * PP + (64 * m) % N1 + lane] = make_float2(a[m], b[m]);
+      } else if constexpr (NTH % N1 == 0 && (N1 * NC / 2) % NTH == 0) {
         float a[NLD], b[NLD];
         plane_pairs_load<N1, NC, NTH, NLD>(sp, t, cstr, is, a, b);
         float2 *lp = L + (t / N1) * PP + t % N1;
@@ -890,7 +909,10 @@ __global__ __launch_bounds__((plane_threads<N1, NC>()), (plane_waves<plane_threa
       }
     }
     if (sp) split0(t, w0, w1);
-    __syncthreads();
+    if constexpr (WAVEIN)
+      wave_sync();
+    else
+      __syncthreads();
     {
       // FFT along k1 of z = col 2 pr + i col 2 pr + 1, then rows k <= HS:
       // col 2 pr gets A = (Z[k] + conj Z[-k]) / 2, col 2 pr + 1 gets
