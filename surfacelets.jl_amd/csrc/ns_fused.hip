@@ -599,8 +599,17 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
   // part row HS's.
   // Only live tiles are read: the pencil pass writes nothing else, and the
   // row pass drops dead tiles whatever they hold.
+  // When the row slots of a wave are exactly the column pairs of its column
+  // pass (same stride, same lanes per line), a wave touches no LDS of another
+  // wave between the barrier after the column pass's transposed reads and
+  // the row pass's transposed writes, and row HS (UW) is staged by wave 0,
+  // which alone reads it: no block barrier starts a subband but the first.
+  constexpr bool OWN = PS == PP && RC2 == R2 && (NC / 2) * R2 == NTH && NC <= 128;
+  constexpr int UHN = OWN ? (NC + 63) / 64 : 1;  // row HS columns per wave-0 lane
   float2 u[RC1];
-  float2 uh = make_float2(0.0f, 0.0f);
+  float2 uh[UHN];
+#pragma unroll
+  for (int h = 0; h < UHN; ++h) uh[h] = make_float2(0.0f, 0.0f);
   {
     const int tid = (int)threadIdx.x;
     const int slot = tid / RC2, j = tid % RC2;
@@ -613,7 +622,15 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
     }
     static_assert(NTH >= NC, "one thread per column of row HS");
     const uint32_t lh = masked ? live[(size_t)jb.slab * H1 + HS] : 0xffffffffu;
-    if (tid < NC && ((lh >> (tid / tpw)) & 1u)) uh = Ub[(unsigned)(HS * NC + tid)];
+    if constexpr (OWN) {
+#pragma unroll
+      for (int h = 0; h < UHN; ++h) {
+        const int kc = tid + 64 * h;
+        if (tid < 64 && kc < NC && ((lh >> (kc / tpw)) & 1u)) uh[h] = Ub[(unsigned)(HS * NC + kc)];
+      }
+    } else {
+      if (tid < NC && ((lh >> (tid / tpw)) & 1u)) uh[0] = Ub[(unsigned)(HS * NC + tid)];
+    }
   }
   for (int p = jb.p0; p < jb.p1; ++p) {
     // Index math is redone per subband from an opaque copy of the thread
@@ -630,13 +647,24 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
       const unsigned o = (unsigned)(row * NC + j + RC2 * r);
       wa[r] = wab[wunit ? 0u : o];
     }
-    if (tid < NC) UW[tid] = wunit ? uh : cscale(uh, wab[(unsigned)(HS * NC + tid)]);
+    if constexpr (OWN) {
+#pragma unroll
+      for (int h = 0; h < UHN; ++h) {
+        const int kc = tid + 64 * h;
+        if (tid < 64 && kc < NC) UW[kc] = wunit ? uh[h] : cscale(uh[h], wab[(unsigned)(HS * NC + kc)]);
+      }
+    } else {
+      if (tid < NC) UW[tid] = wunit ? uh[0] : cscale(uh[0], wab[(unsigned)(HS * NC + tid)]);
+    }
     if (p == jb.p0) {
       RegFFT<N1, 1>::fill_twiddles(tw1, tw1g, tid, NTH);
       RegFFT<NC, 1>::fill_twiddles(twc, twcg, tid, NTH);
       for (int k = tid; k < H1; k += NTH) rl[k] = masked ? live[(size_t)jb.slab * H1 + k] : 0xffffffffu;
     }
-    __syncthreads();  // also: the previous subband is done with L
+    if (!OWN || p == jb.p0)
+      __syncthreads();  // also: the previous subband is done with L
+    else
+      wave_sync();
     const uint32_t lrow = onslot ? rl[slot] : 0u, lhs = slot == 0 ? rl[HS] : 0u;
     const bool act = (lrow | lhs) != 0u;
     float2 *pb = L + row * PS;
