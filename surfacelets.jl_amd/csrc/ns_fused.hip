@@ -404,6 +404,10 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
   const int quarter = tid / QT, lt = tid % QT;
   const int j = (lt / PL) % R2, pq = lt % PL + PL * (lt / 64);
   const int rs = h1 * nt;
+  // G of the group last used (slabs of one group are adjacent in the list and
+  // share it).
+  int gcur = -1;
+  float gv[E];
   for (int s0 = 0; s0 < ns; s0 += NQ) {
     bool lv[NQ];
     bool any = false;
@@ -444,12 +448,20 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
     for (int h = 0; h < NQ; ++h) {
       if (!lv[h]) continue;
       const int4 sl = slist[s0 + h];
-      const float *Gk = ns_group_table(mv, sl.y, false) + kb;
+      if (sl.y != gcur) {
+        const float *Gk = ns_group_table(mv, sl.y, false) + kb;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const TE t(tid, m, c0, n3);
+          gv[m] = in_tile(m) ? Gk[t.off] : 0.0f;
+        }
+        gcur = sl.y;
+      }
       const PlaneRef wb = wedge_b_ref(mv, items[sl.z], k1, false);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const TE t(tid, m, c0, n3);
-        if (in_tile(m)) a[m] = fma2(buf[h][t.pp * PS + t.kp], Gk[t.off] * wb.at(t.k2, t.k3), a[m]);
+        if (in_tile(m)) a[m] = fma2(buf[h][t.pp * PS + t.kp], gv[m] * wb.at(t.k2, t.k3), a[m]);
       }
     }
     __syncthreads();
