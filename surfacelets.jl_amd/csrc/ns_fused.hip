@@ -652,12 +652,17 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
     const int slot = tid / RC2, j = tid % RC2;
     const bool onslot = slot < HS;
     const int row = onslot ? slot : 0;
-    const float *wab = wunit ? mv.planes : mv.planes + jb.pa0 + (p - jb.p0) * jb.pa_step;
+    const float *wab = mv.planes + (wunit ? 0 : jb.pa0 + (p - jb.p0) * jb.pa_step);
+    // Block-uniform branch, one offset per thread and immediate steps (no
+    // per-element selects or 64-bit address arithmetic).
     float wa[RC1];
+    if (wunit) {
 #pragma unroll
-    for (int r = 0; r < RC1; ++r) {
-      const unsigned o = (unsigned)(row * NC + j + RC2 * r);
-      wa[r] = wab[wunit ? 0u : o];
+      for (int r = 0; r < RC1; ++r) wa[r] = 1.0f;
+    } else {
+      const unsigned o0 = 4u * (unsigned)(row * NC + j);
+#pragma unroll
+      for (int r = 0; r < RC1; ++r) wa[r] = at_b(wab, o0 + 4u * RC2 * r);
     }
     if constexpr (OWN) {
 #pragma unroll
@@ -685,8 +690,13 @@ __global__ __launch_bounds__((ana_threads<N1, NC>()), (plane_waves<ana_threads<N
       if (act) {
         // Dead tiles hold +0: their slab values were never loaded (u = 0, uh
         // = 0) and the wedge values are finite and nonnegative.
+        if (wunit) {
 #pragma unroll
-        for (int r = 0; r < RC1; ++r) v[r] = cscale(u[r], wunit ? 1.0f : wa[r]);
+          for (int r = 0; r < RC1; ++r) v[r] = u[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < RC1; ++r) v[r] = cscale(u[r], wa[r]);
+        }
         if (slot == 0) {
 #pragma unroll
           for (int r = 0; r < RC1; ++r) {
