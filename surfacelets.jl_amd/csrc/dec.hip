@@ -1042,14 +1042,20 @@ struct surfhip_s_plan {
     rec->keep.push_back(t);
   }
   // One recorded call per direction (0 analyze, 1 synthesize, 2 adjoint).
+  // `done` is recorded after every replay: an executable graph is destroyed
+  // only once its last replay has finished, never while it may still run.
   struct Cached {
     const void *in = nullptr, *out = nullptr;
     int seen = 0;
     Rec rec;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool pending = false;
   };
   Cached cache[3];
   static void drop(Cached &c) {
+    if (c.pending && c.done) (void)hipEventSynchronize(c.done);
+    c.pending = false;
     if (c.exec) (void)hipGraphExecDestroy(c.exec);
     if (c.rec.g) (void)hipGraphDestroy(c.rec.g);
     c.exec = nullptr;
@@ -1089,14 +1095,20 @@ struct surfhip_s_plan {
       SH_HIP(hipGraphInstantiate(&c.exec, c.rec.g, nullptr, nullptr, 0));
     }
     if (c.exec) {
+      if (!c.done) SH_HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
       SH_HIP(hipGraphLaunch(c.exec, stream));
+      SH_HIP(hipEventRecord(c.done, stream));
+      c.pending = true;
       return;
     }
     body();
   }
 
   ~surfhip_s_plan() {
-    for (Cached &c : cache) drop(c);
+    for (Cached &c : cache) {
+      drop(c);
+      if (c.done) (void)hipEventDestroy(c.done);
+    }
     for (int K = 0; K < 3; ++K) {
       if (kst[K]) (void)hipStreamDestroy(kst[K]);
       if (kjoin[K]) (void)hipEventDestroy(kjoin[K]);

@@ -70,7 +70,10 @@ struct surfhip_ns_batch {
     if (start) (void)hipEventDestroy(start);
     if (origin) (void)hipStreamDestroy(origin);
   }
+  // An executable graph is destroyed only once its last replay (followed by
+  // `done`) has finished, never while it may still run.
   void drop_graph() {
+    if (exec && done) (void)hipEventSynchronize(done);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
