@@ -542,6 +542,32 @@ __device__ __forceinline__ int64_t block_stride(const BlockMap &M) {
   return M.K == 0 ? 1 : (M.K == 1 ? (int64_t)M.sb[0] : (int64_t)M.sb[0] * M.sb[1]);
 }
 
+// f(inner, outer) for idx = tid, tid + DT, ... < n_inner n_outer, with
+// inner = idx mod n_inner and outer = idx / n_inner, without a division per
+// element: one division for the thread's start, then steps with a carry.
+template <class Fn> __device__ __forceinline__ void walk2(int tid, int n_inner, int n_outer, Fn f) {
+  int outer = tid / n_inner, inner = tid - outer * n_inner;
+  const int dq = DT / n_inner, dr = DT - dq * n_inner;
+  while (outer < n_outer) {
+    f(inner, outer);
+    inner += dr;
+    outer += dq;
+    if (inner >= n_inner) {
+      inner -= n_inner;
+      ++outer;
+    }
+  }
+}
+// Element (t, g) of a block's G lines of length n in the order that keeps
+// global accesses coalesced: t fastest along axis 0 (K = 0, lines contiguous
+// in t), g fastest otherwise (adjacent lines adjacent in memory).
+template <class Fn> __device__ __forceinline__ void walk_lines(int K, int tid, int n, int G, Fn f) {
+  if (K == 0)
+    walk2(tid, n, G, [&](int t, int g) { f(t, g); });
+  else
+    walk2(tid, G, n, [&](int g, int t) { f(t, g); });
+}
+
 // Subband lines = c2r along K of S (unnormalised), scattered into the
 // coefficients (element stride os).  G lines per block, two per complex FFT:
 // lines 2f and 2f + 1 have Hermitian spectra A and B, so the inverse of
@@ -565,17 +591,9 @@ __device__ __forceinline__ void c2r_lines_body(const float2 *__restrict__ S, flo
   line_bases(M, l0, valid, LB, tid);
   const int64_t sS = half_stride(M), sC = block_stride(M);
   __syncthreads();
-  for (int idx = tid; idx < hK * G; idx += DT) {
-    int t, g;
-    if (M.K == 0) {
-      g = idx / hK;
-      t = idx - g * hK;
-    } else {
-      t = idx / G;
-      g = idx - t * G;
-    }
+  walk_lines(M.K, tid, hK, G, [&](int t, int g) {
     st[t * G + g] = g < valid ? S[LB.s[g] + t * sS] : make_float2(0.0f, 0.0f);
-  }
+  });
   __syncthreads();
   const bool even = (nK & 1) == 0;
   // Bin t of line g through the Hermitian extension (bins 0 and nK/2 real).
@@ -590,26 +608,17 @@ __device__ __forceinline__ void c2r_lines_body(const float2 *__restrict__ S, flo
     }
     return z;
   };
-  for (int idx = tid; idx < nK * F; idx += DT) {
-    const int t = idx / F, f = idx - t * F;
+  walk2(tid, F, nK, [&](int f, int t) {
     const float2 A = herm(t, 2 * f), B = herm(t, 2 * f + 1);
-    a[idx] = make_float2(A.x - B.y, A.y + B.x);
-  }
+    a[t * F + f] = make_float2(A.x - B.y, A.y + B.x);
+  });
   __syncthreads();
   float2 *res = lds_fft<1, GEN>(a, b, pl, F, tw, tid, DT);
-  for (int idx = tid; idx < nK * G; idx += DT) {
-    int t, g;
-    if (M.K == 0) {
-      g = idx / nK;
-      t = idx - g * nK;
-    } else {
-      t = idx / G;
-      g = idx - t * G;
-    }
-    if (g >= valid) continue;
-    const float2 z = res[t * F + g / 2];
+  walk_lines(M.K, tid, nK, G, [&](int t, int g) {
+    if (g >= valid) return;
+    const float2 z = res[t * F + (g >> 1)];
     out[(LB.c[g] + t * sC) * os] = (g & 1) ? z.y : z.x;
-  }
+  });
 }
 
 template <bool GEN>
@@ -658,37 +667,21 @@ __device__ __forceinline__ void r2c_lines_body(const float *__restrict__ in, int
   const int64_t sS = half_stride(M), sC = block_stride(M);
   __syncthreads();
   float *af = reinterpret_cast<float *>(a);
-  for (int idx = tid; idx < nK * 2 * F; idx += DT) {
-    // Component g & 1 of FFT g / 2 (slot g = G of an odd G is zero).
-    int t, g;
-    if (M.K == 0) {
-      g = idx / nK;
-      t = idx - g * nK;
-    } else {
-      t = idx / (2 * F);
-      g = idx - t * (2 * F);
-    }
-    af[2 * (t * F + g / 2) + (g & 1)] = g < valid ? in[(LB.c[g] + t * sC) * is] : 0.0f;
-  }
+  // Component g & 1 of FFT g / 2 (slot g = G of an odd G is zero).
+  walk_lines(M.K, tid, nK, 2 * F, [&](int t, int g) {
+    af[2 * (t * F + (g >> 1)) + (g & 1)] = g < valid ? in[(LB.c[g] + t * sC) * is] : 0.0f;
+  });
   __syncthreads();
   float2 *res = lds_fft<-1, GEN>(a, b, pl, F, tw, tid, DT);
-  for (int idx = tid; idx < hK * G; idx += DT) {
-    int t, g;
-    if (M.K == 0) {
-      g = idx / hK;
-      t = idx - g * hK;
-    } else {
-      t = idx / G;
-      g = idx - t * G;
-    }
-    if (g >= valid) continue;
-    const int f = g / 2;
-    const float2 Z = res[t * F + f], Zm = res[((nK - t) % nK) * F + f];
+  walk_lines(M.K, tid, hK, G, [&](int t, int g) {
+    if (g >= valid) return;
+    const int f = g >> 1;
+    const float2 Z = res[t * F + f], Zm = res[(t == 0 ? 0 : nK - t) * F + f];
     float2 v = (g & 1) ? make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x))
                        : make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
     if (dbl && t >= 1 && t < nK - t) v = cscale(v, 2.0f);
     S[LB.s[g] + t * sS] = v;
-  }
+  });
 }
 
 template <bool GEN>
