@@ -319,7 +319,7 @@ __global__ __launch_bounds__(pblock_threads<NP>(), 4) void kP_ana(const float2 *
   // The R2 lanes of a pencil lie in one wave (see kP_syn).
   constexpr int PL = 64 / R2;
   static_assert(HT % 64 == 0 && TP % PL == 0, "pencils of a half fill whole waves");
-  const int half = tid / HT, lt = tid % HT;
+  const int half = __builtin_amdgcn_readfirstlane(tid / HT), lt = tid % HT;
   const int j = (lt / PL) % R2, pq = lt % PL + PL * (lt / 64);
   const int rs = h1 * nt;
   for (int q0 = 0; q0 < gd.nB; q0 += 2) {
@@ -401,29 +401,44 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
   // pencils.
   constexpr int PL = 64 / R2;
   static_assert(QT % 64 == 0 && TP % PL == 0, "pencils of a quarter fill whole waves");
-  const int quarter = tid / QT, lt = tid % QT;
+  // A quarter is whole waves, so `quarter` is wave-uniform: the slab list and
+  // live bits it indexes are read with scalar loads.
+  const int quarter = __builtin_amdgcn_readfirstlane(tid / QT), lt = tid % QT;
   const int j = (lt / PL) % R2, pq = lt % PL + PL * (lt / 64);
   const int rs = h1 * nt;
   // G of the group last used (slabs of one group are adjacent in the list and
   // share it).
   int gcur = -1;
   float gv[E];
-  for (int s0 = 0; s0 < ns; s0 += NQ) {
-    bool lv[NQ];
-    bool any = false;
+  // The first step (NQ consecutive slabs of the list) at or after s with a
+  // live slab in this tile, and its live bits; ns when there is none.
+  auto seek = [&](int s, bool(&lv)[NQ]) {
+    for (; s < ns; s += NQ) {
+      bool any = false;
 #pragma unroll
-    for (int h = 0; h < NQ; ++h) {
-      lv[h] = s0 + h < ns && ((live[(size_t)slist[s0 + h].x * h1 + k1] >> tile) & 1u);
-      any = any || lv[h];
+      for (int h = 0; h < NQ; ++h) {
+        lv[h] = s + h < ns && ((live[(size_t)slist[s + h].x * h1 + k1] >> tile) & 1u);
+        any = any || lv[h];
+      }
+      if (any) return s;
     }
-    if (!any) continue;
+    return ns;
+  };
+  // The pencils of this quarter's slab of step s.
+  auto load_step = [&](int s, float2(&v)[F::NV]) {
+    const float2 *src = slabs + (size_t)slist[s + quarter].x * H + (size_t)k1 * nt + c0;
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = src[(unsigned)((j + R2 * r) * rs + pq)];
+  };
+  // Software pipeline over the steps: v is free once a step's transform sits
+  // in LDS, so the next step's pencils are requested before this step's mask
+  // values, and their latency and the mask loads' share one wait.
+  bool lv[NQ];
+  int s0 = seek(0, lv);
+  float2 v[F::NV];
+  if (s0 < ns && lv[quarter]) load_step(s0, v);
+  while (s0 < ns) {
     const bool act = lv[quarter];
-    float2 v[F::NV];
-    if (act) {
-      const float2 *src = slabs + (size_t)slist[s0 + quarter].x * H + (size_t)k1 * nt + c0;
-#pragma unroll
-      for (int r = 0; r < R1; ++r) v[r] = src[(unsigned)((j + R2 * r) * rs + pq)];
-    }
     float2 *pb = buf[quarter] + pq * PS;
     if (act) {
       F::phase1(v, j, tw);
@@ -444,6 +459,9 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
         }
     }
     __syncthreads();
+    bool ln[NQ];
+    const int s1 = seek(s0 + NQ, ln);
+    if (s1 < ns && ln[quarter]) load_step(s1, v);
 #pragma unroll
     for (int h = 0; h < NQ; ++h) {
       if (!lv[h]) continue;
@@ -465,6 +483,9 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
       }
     }
     __syncthreads();
+    s0 = s1;
+#pragma unroll
+    for (int h = 0; h < NQ; ++h) lv[h] = ln[h];
   }
   if (OR == AX2) {
 #pragma unroll
@@ -480,7 +501,6 @@ __global__ __launch_bounds__(syn_threads<NP>(), 4) void kP_syn(const float2 *__r
     if (in_tile(m)) buf[0][t.pp * PS + t.kp] = a[m];
   }
   __syncthreads();
-  float2 v[F::NV];
   tile_fft<NP, 1, true>(buf[0] + pq * PS, tw, j, quarter == 0, v);
   if (quarter == 0) {
     float2 *dst = V + (size_t)k1 * n2 + c0;
