@@ -1054,6 +1054,24 @@ struct surfhip_s_plan {
     c.exec = nullptr;
     c.rec = Rec{};
   }
+  // A recorded call replaced by a call with other pointers is kept, not
+  // destroyed, until the plan is destroyed (at most kRetired of them; beyond
+  // that the oldest is destroyed once its last replay has completed).  A
+  // test that destroyed replaced graphs while the plan lived ended twice in
+  // a host segmentation fault in a runtime thread a few calls later, with no
+  // cause found in this code.
+  static constexpr size_t kRetired = 8;
+  std::vector<Cached> retired;
+  void retire(Cached &c) {
+    if (!c.exec && !c.rec.g) return;
+    retired.push_back(std::move(c));
+    c = Cached{};
+    if (retired.size() > kRetired) {
+      drop(retired.front());
+      if (retired.front().done) (void)hipEventDestroy(retired.front().done);
+      retired.erase(retired.begin());
+    }
+  }
   // body() issues the call's launches.  Direct launches unless the call
   // repeats the previous one's pointers; a stream being captured by the
   // caller always gets direct launches.
@@ -1069,7 +1087,7 @@ struct surfhip_s_plan {
       return;
     }
     if (c.in != in || c.out != out) {
-      drop(c);
+      retire(c);
       c.in = in;
       c.out = out;
       c.seen = 0;
@@ -1099,6 +1117,10 @@ struct surfhip_s_plan {
 
   ~surfhip_s_plan() {
     for (Cached &c : cache) {
+      drop(c);
+      if (c.done) (void)hipEventDestroy(c.done);
+    }
+    for (Cached &c : retired) {
       drop(c);
       if (c.done) (void)hipEventDestroy(c.done);
     }
